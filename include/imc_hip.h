@@ -1,0 +1,119 @@
+/*
+ * imc_hip.h -- C ABI of libimc_hip, the MI355X (gfx950) particle engine that stands in
+ * for the per-timestep particle loop of MixedPrecisionIMC.jl.
+ *
+ * The reference has no FFI layer: its hot path is five module functions that main()
+ * calls positionally, each mutating its arguments (src/MixedPrecisionIMC.jl:139-146 in
+ * 1D, :168-171 in 2D).  Each entry point below replaces one of them; the citation is
+ * given on each declaration.  Conventions:
+ *   - every function returns IMC_OK (0) or a negative IMC_E* status; imc_last_error()
+ *     gives the message of the last failure on that context;
+ *   - host arrays are FP64 and Julia column-major (cell (ix,iy) at ix + nx*iy, energy
+ *     scale k at + nx*ny*k), copied in and out; the library owns all device memory
+ *     (particle structure-of-arrays, Philox key state, tally scratch);
+ *   - one context per host thread; calls are synchronous on the context's HIP stream.
+ */
+#ifndef IMC_HIP_H
+#define IMC_HIP_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define IMC_OK 0
+#define IMC_EINVAL (-1)     /* bad argument or call order */
+#define IMC_EHIP (-2)       /* HIP runtime failure */
+#define IMC_ENOMEM (-3)     /* device allocation failed */
+#define IMC_ENUMERIC (-4)   /* non-finite value detected (NaN/Inf opacity, energy) */
+
+#define IMC_VACUUM 0
+#define IMC_REFLECT 1
+
+/* Reference quirks that can be reproduced on request (product default: fixed). */
+#define IMC_QUIRK_Q1_RADSRC_NBODY 1  /* 2D radiation-source births counted with n_body (imc_sourcing.jl:357) */
+#define IMC_QUIRK_Q2_LR_FACE_DX 2    /* 2D left/right face births use dx[j] for y (imc_sourcing.jl:301,316) */
+
+typedef struct imc_ctx imc_ctx;
+
+typedef struct {
+  int32_t geometry;          /* 1 or 2 (deck GEOMETRY) */
+  int32_t precision;         /* 64: FP64 tracking+weights+tallies; 32: FP32 tracking/weights with
+                                compensated FP32 tallies; 16: FP16 tracking, FP32 weights,
+                                compensated FP32 tallies (deck PRECISION) */
+  int32_t nx, ny;            /* cells; ny = 1 in 1D */
+  int32_t n_scales;          /* K = length(energyscales), 1..16 */
+  int32_t quirks;            /* IMC_QUIRK_* */
+  int32_t bc[4];             /* LEFTBC, RIGHTBC, TOPBC, BOTTOMBC (MixedPrecisionIMC.jl:121-122,158-162) */
+  uint32_t seed;             /* deck SEED: Philox key word 0 */
+  int32_t device;            /* HIP device ordinal */
+  double energyscales[16];   /* descending (imc_mesh.jl:130) */
+  double phys_c, phys_a, distancescale;  /* constants.jl:9-12, imc_mesh.jl:131 */
+  int64_t capacity;          /* initial particle capacity; grows on demand */
+  int32_t reserved[8];
+} imc_config;
+
+/* Per-call timing (HIP events on the context stream), accumulated since the last reset. */
+typedef struct {
+  double ms_source, ms_transport, ms_clean, ms_tally;
+  double ms_transport_kernel;     /* the transport kernel launches alone */
+  int64_t n_transport_launches;
+  int64_t histories;              /* particles handed to transport */
+  int64_t census_out;             /* particles written back to census */
+  int64_t segments;               /* event-loop passes (the reference's `iterations`) */
+} imc_timing;
+
+/* Creates a context: MixedPrecisionIMC.jl:117 (particles = Vector{Vector{}}()) plus the
+ * simvars/mesh scalars the hot path reads. */
+int imc_create(imc_ctx** ctx, const imc_config* cfg);
+int imc_destroy(imc_ctx* ctx);
+const char* imc_last_error(const imc_ctx* ctx);
+
+/* Cell widths (mesh.dx, mesh.dy from imc_mesh.jl:42-173).  dy is ignored in 1D. */
+int imc_set_geometry(imc_ctx* ctx, const double* dx, const double* dy);
+
+/* Per-step coefficient upload: the outputs of Update.update (imc_update.jl:12-70) that
+ * the hot path reads -- sigma_a[:,1], sigma_s[:,1], fleck -- plus mesh.temp, mesh.radsource
+ * and mesh.temp_surf.  step is the 0-based timestep index (Philox key word 1).
+ * t_surf: 1D -> t_surf[0][0] left, t_surf[1][0] right; 2D -> bottom(nx), top(nx),
+ * left(ny), right(ny) as in imc_mesh.jl:222-259. */
+int imc_set_step_fields(imc_ctx* ctx, int64_t step, double dt, const double* sigma_a,
+                        const double* sigma_s, const double* fleck, const double* temp,
+                        const double* radsource, const double* const* t_surf);
+
+/* Sourcing.sourcing(mesh, simvars, particles) -- imc_sourcing.jl:12-370.
+ * Writes mesh.emittedenergy (C x K), mesh.totalenergy and the population after births. */
+int imc_source(imc_ctx* ctx, int64_t n_input, int64_t n_max, double cellmin,
+               double* emittedenergy, double* totalenergy, int64_t* n_alive);
+
+/* Transport.MC / Transport.MC2D(mesh, simvars, particles) -- imc_transport.jl:13-210 and
+ * :483-732.  energydep (C x K, per unit length/area, scaled) is overwritten; *lostenergy
+ * receives this call's vacuum loss (caller adds, as mesh.lostenergy +=); *segments the
+ * number of event-loop passes (simvars.iterations increment). */
+int imc_transport(imc_ctx* ctx, double* energydep, double* lostenergy, int64_t* segments);
+
+/* Clean.clean(particles) -- imc_clean.jl:6-19: order-preserving removal of killed
+ * particles.  Also forms the census radiation energy density for imc_tally_census. */
+int imc_clean(imc_ctx* ctx, int64_t* n_alive);
+
+/* Census part of Tally.tally -- imc_tally.jl:157-190: radenergydens (C). */
+int imc_tally_census(imc_ctx* ctx, double* radenergydens);
+
+/* Particle records in the reference's AoS layout (imc_sourcing.jl:173 / :276):
+ * 1D [origin, t, cell, x, mu, nu, E, E0, scale], 2D [t, ix, iy, x, y, theta, nu, E, E0, scale],
+ * 1-based cells.  Export writes at most cap records; import replaces the population. */
+int imc_particle_count(imc_ctx* ctx, int64_t* n);
+int imc_export_particles(imc_ctx* ctx, double* records, int64_t cap, int64_t* n);
+int imc_import_particles(imc_ctx* ctx, const double* records, int64_t n);
+
+int imc_get_timing(imc_ctx* ctx, imc_timing* out);
+int imc_reset_timing(imc_ctx* ctx);
+int imc_synchronize(imc_ctx* ctx);
+
+/* Bytes of particle state per particle held between kernels (S_state). */
+int imc_state_bytes(const imc_ctx* ctx, int32_t* bytes);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,30 @@
+"""Sourcing.sourcing -- src/imc_sourcing.jl:12-370, executed by libimc_hip.
+
+The host uploads the step's coefficients (outputs of Update.update) and the engine computes
+the emission, radiation-source and surface energies, the particle budget, the per-cell counts
+and the births on the GPU.  Mutates mesh.emittedenergy, mesh.totalenergy and the particle
+population, as the reference does.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+from ._hip import Engine
+
+
+def upload_step_fields(mesh, simvars, particles: Engine) -> None:
+    """imc_set_step_fields: sigma_a[:,1], sigma_s[:,1], fleck, temp, radsource, temp_surf."""
+    if mesh.geometry == "1D":
+        tsurf = [np.array([mesh.temp_surf[0]]), np.array([mesh.temp_surf[1]])]
+    else:
+        tsurf = [np.asarray(v, dtype=np.float64) for v in mesh.temp_surf]
+    particles.set_step_fields(simvars.stepindex, simvars.dt, mesh.sigma_a[..., 0], mesh.sigma_s[..., 0],
+                              mesh.fleck, mesh.temp, mesh.radsource, tsurf)
+
+
+def sourcing(mesh, simvars, particles: Engine) -> None:
+    upload_step_fields(mesh, simvars, particles)
+    emitted, etot, n = particles.source(simvars.n_input, simvars.n_max, simvars.cellmin)
+    K = len(mesh.energyscales)
+    mesh.emittedenergy = emitted.reshape(mesh.shape + (K,), order="F")
+    mesh.totalenergy = etot

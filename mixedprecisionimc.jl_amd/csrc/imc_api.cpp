@@ -1,0 +1,518 @@
+// imc_api.cpp -- C ABI of libimc_hip: context, device memory and the entry points that
+// replace Sourcing.sourcing / Transport.MC* / Clean.clean / the census tally of
+// MixedPrecisionIMC.jl.  Kernels live in imc_kernels.hip.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstddef>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <vector>
+
+#include "imc_device.h"
+#include "imc_internal.h"
+
+using namespace imc;
+
+namespace {
+
+int fail(Ctx& c, int code, const char* what) {
+  c.err = what;
+  return code;
+}
+int hipfail(Ctx& c, hipError_t e, const char* where) {
+  char buf[256];
+  std::snprintf(buf, sizeof(buf), "%s: %s", where, hipGetErrorString(e));
+  c.err = buf;
+  return IMC_EHIP;
+}
+
+#define HIPCHK(expr, where)                          \
+  do {                                               \
+    hipError_t e_ = (expr);                          \
+    if (e_ != hipSuccess) return hipfail(c, e_, where); \
+  } while (0)
+
+size_t tsize(int prec) { return prec == 64 ? 8 : (prec == 32 ? 4 : 2); }
+size_t wsize(int prec) { return prec == 64 ? 8 : 4; }
+
+template <class P>
+hipError_t dalloc(P** p, size_t bytes) {
+  return hipMalloc((void**)p, std::max<size_t>(bytes, 8));
+}
+
+void free_soa(SoA& s) {
+  void* ps[] = {s.cellk, s.x, s.y, s.mu, s.t, s.E, s.E0};
+  for (void* p : ps) (void)hipFree(p);
+  s = SoA{};
+}
+
+hipError_t alloc_soa(Ctx& c, SoA& s, int64_t cap) {
+  const size_t T = tsize(c.prec), W = wsize(c.prec), n = (size_t)std::max<int64_t>(cap, 1);
+  hipError_t e;
+  if ((e = dalloc(&s.cellk, n * 4)) != hipSuccess) return e;
+  if ((e = dalloc(&s.x, n * T)) != hipSuccess) return e;
+  if (c.is2d && (e = dalloc(&s.y, n * T)) != hipSuccess) return e;
+  if ((e = dalloc(&s.mu, n * T)) != hipSuccess) return e;
+  if ((e = dalloc(&s.t, n * T)) != hipSuccess) return e;
+  if ((e = dalloc(&s.E, n * W)) != hipSuccess) return e;
+  if ((e = dalloc(&s.E0, n * W)) != hipSuccess) return e;
+  s.cap = cap;
+  return hipSuccess;
+}
+
+hipError_t copy_soa(Ctx& c, SoA& dst, const SoA& src, int64_t n) {
+  if (n <= 0) return hipSuccess;
+  const size_t T = tsize(c.prec), W = wsize(c.prec);
+  hipError_t e;
+  if ((e = hipMemcpyAsync(dst.cellk, src.cellk, n * 4, hipMemcpyDeviceToDevice, c.stream)) != hipSuccess) return e;
+  if ((e = hipMemcpyAsync(dst.x, src.x, n * T, hipMemcpyDeviceToDevice, c.stream)) != hipSuccess) return e;
+  if (c.is2d && (e = hipMemcpyAsync(dst.y, src.y, n * T, hipMemcpyDeviceToDevice, c.stream)) != hipSuccess) return e;
+  if ((e = hipMemcpyAsync(dst.mu, src.mu, n * T, hipMemcpyDeviceToDevice, c.stream)) != hipSuccess) return e;
+  if ((e = hipMemcpyAsync(dst.t, src.t, n * T, hipMemcpyDeviceToDevice, c.stream)) != hipSuccess) return e;
+  if ((e = hipMemcpyAsync(dst.E, src.E, n * W, hipMemcpyDeviceToDevice, c.stream)) != hipSuccess) return e;
+  return hipMemcpyAsync(dst.E0, src.E0, n * W, hipMemcpyDeviceToDevice, c.stream);
+}
+
+// Grows both particle buffers to hold at least `need` particles, keeping [0, keep) of
+// the current buffer.
+int grow(Ctx& c, int64_t need, int64_t keep) {
+  if (need <= c.buf[c.cur].cap) return IMC_OK;
+  if (need > (int64_t)0xFFFFFFF0ll) return fail(c, IMC_EINVAL, "particle population exceeds 2^32 slots");
+  const int64_t cap = std::max<int64_t>(need + need / 4, 2 * c.buf[c.cur].cap);
+  SoA fresh;
+  hipError_t e = alloc_soa(c, fresh, cap);
+  if (e != hipSuccess) { free_soa(fresh); return fail(c, IMC_ENOMEM, "particle buffer allocation failed"); }
+  HIPCHK(copy_soa(c, fresh, c.buf[c.cur], keep), "grow copy");
+  HIPCHK(hipStreamSynchronize(c.stream), "grow sync");
+  free_soa(c.buf[c.cur]);
+  c.buf[c.cur] = fresh;
+  free_soa(c.buf[c.cur ^ 1]);
+  e = alloc_soa(c, c.buf[c.cur ^ 1], cap);
+  if (e != hipSuccess) return fail(c, IMC_ENOMEM, "particle buffer allocation failed");
+  (void)hipFree(c.d_tile);
+  c.tile_len = cap / 2048 + 2;
+  e = dalloc(&c.d_tile, sizeof(int64_t) * c.tile_len);
+  if (e != hipSuccess) return fail(c, IMC_ENOMEM, "tile buffer allocation failed");
+  return IMC_OK;
+}
+
+int fetch_scalars(Ctx& c) {
+  HIPCHK(hipMemcpyAsync(&c.h_sc, c.d_sc, sizeof(Scalars), hipMemcpyDeviceToHost, c.stream), "scalars D2H");
+  HIPCHK(hipStreamSynchronize(c.stream), "scalars sync");
+  return IMC_OK;
+}
+
+float elapsed(hipEvent_t a, hipEvent_t b) {
+  float ms = 0.f;
+  (void)hipEventSynchronize(b);
+  (void)hipEventElapsedTime(&ms, a, b);
+  return ms;
+}
+
+}  // namespace
+
+namespace imc {
+
+int state_bytes(int prec, bool is2d) {
+  const int T = (int)tsize(prec), W = (int)wsize(prec);
+  return 4 + (is2d ? 4 : 3) * T + 2 * W;
+}
+
+Geo make_geo(const Ctx& c) {
+  Geo g;
+  g.nx = c.nx; g.ny = c.ny; g.C = c.C; g.K = c.K; g.NB = c.NB;
+  for (int i = 0; i < 4; ++i) g.bc[i] = c.cfg.bc[i];
+  g.seed = c.cfg.seed; g.step = (uint32_t)c.step;
+  g.dt = c.dt; g.c = c.cfg.phys_c; g.s = c.cfg.distancescale;
+  g.dxs = c.d_dxs; g.dys = c.d_dys; g.dxw = c.d_dxw; g.dyw = c.d_dyw;
+  g.sigcol = c.d_sigcol; g.sigdep = c.d_sigdep;
+  for (int k = 0; k < 16; ++k) g.scales[k] = c.cfg.energyscales[k];
+  return g;
+}
+
+}  // namespace imc
+
+extern "C" {
+
+int imc_create(imc_ctx** out, const imc_config* cfg) {
+  if (!out || !cfg) return IMC_EINVAL;
+  *out = nullptr;
+  imc_ctx* h = new (std::nothrow) imc_ctx();
+  if (!h) return IMC_ENOMEM;
+  Ctx& c = h->c;
+  c.cfg = *cfg;
+  *out = h;
+  if (cfg->geometry != 1 && cfg->geometry != 2) return fail(c, IMC_EINVAL, "geometry must be 1 or 2");
+  if (cfg->precision != 64 && cfg->precision != 32 && cfg->precision != 16)
+    return fail(c, IMC_EINVAL, "precision must be 64, 32 or 16");
+  if (cfg->nx < 1 || (cfg->geometry == 2 && cfg->ny < 1)) return fail(c, IMC_EINVAL, "nx/ny must be >= 1");
+  if (cfg->n_scales < 1 || cfg->n_scales > 16) return fail(c, IMC_EINVAL, "n_scales must be in 1..16");
+  for (int i = 0; i < 4; ++i)
+    if (cfg->bc[i] != IMC_VACUUM && cfg->bc[i] != IMC_REFLECT) return fail(c, IMC_EINVAL, "bc must be VACUUM or REFLECT");
+  c.prec = cfg->precision;
+  c.is2d = cfg->geometry == 2;
+  c.nx = cfg->nx;
+  c.ny = c.is2d ? cfg->ny : 1;
+  c.C = c.nx * c.ny;
+  if ((int64_t)c.nx * c.ny >= (1ll << KSHIFT)) return fail(c, IMC_EINVAL, "too many cells (limit 2^28)");
+  c.K = cfg->n_scales;
+  c.NB = c.C * c.K + 1;
+  c.NE = c.is2d ? (int64_t)2 * c.nx + 2 * c.ny + 2 * (int64_t)c.C : 2 + 2 * (int64_t)c.C;
+  c.device = cfg->device;
+  HIPCHK(hipSetDevice(c.device), "hipSetDevice");
+  HIPCHK(hipDeviceGetAttribute(&c.num_cu, hipDeviceAttributeMultiprocessorCount, c.device), "CU count");
+  HIPCHK(hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking), "stream");
+  HIPCHK(hipEventCreate(&c.ev0), "event"); HIPCHK(hipEventCreate(&c.ev1), "event");
+  HIPCHK(hipEventCreate(&c.evk0), "event"); HIPCHK(hipEventCreate(&c.evk1), "event");
+  const size_t T = tsize(c.prec), W = wsize(c.prec), C = (size_t)c.C;
+  const size_t nx = (size_t)c.nx, ny = (size_t)c.ny;
+  bool ok = true;
+  ok &= dalloc(&c.d_dx, 8 * nx) == hipSuccess;
+  ok &= dalloc(&c.d_dy, 8 * ny) == hipSuccess;
+  ok &= dalloc(&c.d_dxs, T * nx) == hipSuccess;
+  ok &= dalloc(&c.d_dys, T * ny) == hipSuccess;
+  ok &= dalloc(&c.d_dxw, W * nx) == hipSuccess;
+  ok &= dalloc(&c.d_dyw, W * ny) == hipSuccess;
+  ok &= dalloc(&c.d_temp, 8 * C) == hipSuccess;
+  ok &= dalloc(&c.d_fleck, 8 * C) == hipSuccess;
+  ok &= dalloc(&c.d_sa, 8 * C) == hipSuccess;
+  ok &= dalloc(&c.d_ss, 8 * C) == hipSuccess;
+  ok &= dalloc(&c.d_radsrc, 8 * C) == hipSuccess;
+  ok &= dalloc(&c.d_tsurf, 8 * (c.is2d ? 2 * nx + 2 * ny : 2)) == hipSuccess;
+  ok &= dalloc(&c.d_sigcol, T * C) == hipSuccess;
+  ok &= dalloc(&c.d_sigdep, W * C) == hipSuccess;
+  const size_t NE = (size_t)c.NE;
+  ok &= dalloc(&c.d_ent_e, 8 * NE) == hipSuccess;
+  ok &= dalloc(&c.d_ent_esc, 8 * NE) == hipSuccess;
+  ok &= dalloc(&c.d_ent_term, 8 * NE) == hipSuccess;
+  ok &= dalloc(&c.d_ent_k, 4 * NE) == hipSuccess;
+  ok &= dalloc(&c.d_ent_cnt, 8 * NE) == hipSuccess;
+  ok &= dalloc(&c.d_ent_off, 8 * NE) == hipSuccess;
+  c.red_len = 2 * ((int64_t)NE / 256 + 4);
+  ok &= dalloc(&c.d_red, 8 * (size_t)c.red_len) == hipSuccess;
+  ok &= dalloc(&c.d_emitted, 8 * C * c.K) == hipSuccess;
+  ok &= dalloc(&c.d_dep, 8 * (size_t)c.NB) == hipSuccess;
+  ok &= dalloc(&c.d_rad, 8 * C) == hipSuccess;
+  ok &= dalloc(&c.d_sc, sizeof(Scalars)) == hipSuccess;
+  if (!ok) return fail(c, IMC_ENOMEM, "device allocation failed");
+  // tally partials: sized for the larger of transport (NB bins) and clean (C bins)
+  int gt, tmt, gc, tmc; size_t lt, lc;
+  transport_choose(c, &gt, &tmt, &lt);
+  clean_choose(c, &gc, &tmc, &lc);
+  const size_t acc = c.prec == 64 ? 8 : 4;
+  c.partial_bytes = std::max((size_t)gt * c.NB * 2 * acc, (size_t)gc * C * 2 * acc);
+  if (dalloc(&c.d_partial, c.partial_bytes) != hipSuccess) return fail(c, IMC_ENOMEM, "tally partial allocation failed");
+  const int64_t cap = std::max<int64_t>(cfg->capacity, 4096);
+  if (alloc_soa(c, c.buf[0], cap) != hipSuccess || alloc_soa(c, c.buf[1], cap) != hipSuccess)
+    return fail(c, IMC_ENOMEM, "particle buffer allocation failed");
+  c.tile_len = cap / 2048 + 2;
+  if (dalloc(&c.d_tile, sizeof(int64_t) * c.tile_len) != hipSuccess) return fail(c, IMC_ENOMEM, "tile allocation failed");
+  std::memset(&c.h_sc, 0, sizeof(Scalars));
+  HIPCHK(hipMemcpyAsync(c.d_sc, &c.h_sc, sizeof(Scalars), hipMemcpyHostToDevice, c.stream), "scalars H2D");
+  HIPCHK(hipStreamSynchronize(c.stream), "create sync");
+  return IMC_OK;
+}
+
+int imc_destroy(imc_ctx* h) {
+  if (!h) return IMC_OK;
+  Ctx& c = h->c;
+  if (c.stream) (void)hipStreamSynchronize(c.stream);
+  free_soa(c.buf[0]); free_soa(c.buf[1]);
+  void* ptrs[] = {c.d_dx, c.d_dy, c.d_dxs, c.d_dys, c.d_dxw, c.d_dyw, c.d_temp, c.d_fleck, c.d_sa, c.d_ss,
+                  c.d_radsrc, c.d_tsurf, c.d_sigcol, c.d_sigdep, c.d_ent_e, c.d_ent_esc, c.d_ent_term,
+                  c.d_ent_k, c.d_ent_cnt, c.d_ent_off, c.d_red, c.d_emitted, c.d_dep, c.d_rad, c.d_sc,
+                  c.d_partial, c.d_tile};
+  for (void* p : ptrs) (void)hipFree(p);
+  hipEvent_t evs[] = {c.ev0, c.ev1, c.evk0, c.evk1};
+  for (hipEvent_t e : evs)
+    if (e) (void)hipEventDestroy(e);
+  if (c.stream) (void)hipStreamDestroy(c.stream);
+  delete h;
+  return IMC_OK;
+}
+
+const char* imc_last_error(const imc_ctx* h) { return h ? h->c.err.c_str() : "null context"; }
+
+int imc_set_geometry(imc_ctx* h, const double* dx, const double* dy) {
+  if (!h) return IMC_EINVAL;
+  Ctx& c = h->c;
+  if (!dx || (c.is2d && !dy)) return fail(c, IMC_EINVAL, "dx (and dy in 2D) required");
+  for (int i = 0; i < c.nx; ++i)
+    if (!(dx[i] > 0.0) || !std::isfinite(dx[i])) return fail(c, IMC_EINVAL, "dx must be positive and finite");
+  std::vector<double> ones(c.ny, 1.0);
+  const double* ydata = c.is2d ? dy : ones.data();
+  for (int j = 0; j < c.ny; ++j)
+    if (!(ydata[j] > 0.0) || !std::isfinite(ydata[j])) return fail(c, IMC_EINVAL, "dy must be positive and finite");
+  HIPCHK(hipMemcpyAsync(c.d_dx, dx, 8 * c.nx, hipMemcpyHostToDevice, c.stream), "dx H2D");
+  HIPCHK(hipMemcpyAsync(c.d_dy, ydata, 8 * c.ny, hipMemcpyHostToDevice, c.stream), "dy H2D");
+  HIPCHK(hipStreamSynchronize(c.stream), "geometry sync");
+  c.geo_set = true;
+  return IMC_OK;
+}
+
+int imc_set_step_fields(imc_ctx* h, int64_t step, double dt, const double* sigma_a, const double* sigma_s,
+                        const double* fleck, const double* temp, const double* radsource,
+                        const double* const* t_surf) {
+  if (!h) return IMC_EINVAL;
+  Ctx& c = h->c;
+  if (!c.geo_set) return fail(c, IMC_EINVAL, "imc_set_geometry must be called first");
+  if (!sigma_a || !sigma_s || !fleck || !temp || !radsource || !t_surf)
+    return fail(c, IMC_EINVAL, "null step field");
+  if (step < 0 || step > 0xFFFFFFFFll) return fail(c, IMC_EINVAL, "step out of range");
+  if (!(dt > 0.0) || !std::isfinite(dt)) return fail(c, IMC_EINVAL, "dt must be positive and finite");
+  for (int i = 0; i < c.C; ++i) {
+    if (!std::isfinite(sigma_a[i]) || !std::isfinite(sigma_s[i]) || !std::isfinite(fleck[i]) ||
+        !std::isfinite(temp[i]) || !std::isfinite(radsource[i]) || sigma_a[i] < 0.0 || sigma_s[i] < 0.0)
+      return fail(c, IMC_ENUMERIC, "non-finite or negative opacity / Fleck factor / temperature");
+    if (!(fleck[i] >= 0.0 && fleck[i] <= 1.0)) return fail(c, IMC_ENUMERIC, "Fleck factor outside [0, 1]");
+  }
+  std::vector<double> ts;
+  if (c.is2d) {
+    for (int q = 0; q < 4; ++q) {
+      const int len = q < 2 ? c.nx : c.ny;
+      if (!t_surf[q]) return fail(c, IMC_EINVAL, "null t_surf entry");
+      ts.insert(ts.end(), t_surf[q], t_surf[q] + len);
+    }
+  } else {
+    if (!t_surf[0] || !t_surf[1]) return fail(c, IMC_EINVAL, "null t_surf entry");
+    ts = {t_surf[0][0], t_surf[1][0]};
+  }
+  c.step = step;
+  c.dt = dt;
+  const size_t C = (size_t)c.C;
+  HIPCHK(hipMemcpyAsync(c.d_sa, sigma_a, 8 * C, hipMemcpyHostToDevice, c.stream), "sigma_a H2D");
+  HIPCHK(hipMemcpyAsync(c.d_ss, sigma_s, 8 * C, hipMemcpyHostToDevice, c.stream), "sigma_s H2D");
+  HIPCHK(hipMemcpyAsync(c.d_fleck, fleck, 8 * C, hipMemcpyHostToDevice, c.stream), "fleck H2D");
+  HIPCHK(hipMemcpyAsync(c.d_temp, temp, 8 * C, hipMemcpyHostToDevice, c.stream), "temp H2D");
+  HIPCHK(hipMemcpyAsync(c.d_radsrc, radsource, 8 * C, hipMemcpyHostToDevice, c.stream), "radsource H2D");
+  HIPCHK(hipMemcpyAsync(c.d_tsurf, ts.data(), 8 * ts.size(), hipMemcpyHostToDevice, c.stream), "tsurf H2D");
+  HIPCHK(launch_prep(c), "prep kernel");
+  HIPCHK(hipStreamSynchronize(c.stream), "step fields sync");
+  c.step_set = true;
+  return IMC_OK;
+}
+
+int imc_source(imc_ctx* h, int64_t n_input, int64_t n_max, double cellmin, double* emittedenergy,
+               double* totalenergy, int64_t* n_alive) {
+  if (!h) return IMC_EINVAL;
+  Ctx& c = h->c;
+  if (!c.step_set) return fail(c, IMC_EINVAL, "imc_set_step_fields must be called before imc_source");
+  if (n_input < 0 || n_max < 0 || !(cellmin >= 0.0)) return fail(c, IMC_EINVAL, "negative particle budget");
+  HIPCHK(hipEventRecord(c.ev0, c.stream), "event");
+  HIPCHK(launch_source_entries(c), "source entries kernel");
+  HIPCHK(launch_tree_sum(c, c.d_ent_term, c.NE, (double*)((char*)c.d_sc + offsetof(Scalars, E_tot))),
+         "energy sum kernel");
+  HIPCHK(launch_counts_and_scan(c, n_input, n_max, cellmin), "count kernels");
+  int rc = fetch_scalars(c);
+  if (rc) return rc;
+  if (!std::isfinite(c.h_sc.E_tot) || !(c.h_sc.E_tot > 0.0))
+    return fail(c, IMC_ENUMERIC, "total source energy is zero or not finite");
+  const int64_t need = c.h_sc.n_census + c.h_sc.n_births;
+  rc = grow(c, need, c.h_sc.n_census);
+  if (rc) return rc;
+  HIPCHK(launch_births(c, c.h_sc.n_births), "birth kernel");
+  if (emittedenergy)
+    HIPCHK(hipMemcpyAsync(emittedenergy, c.d_emitted, 8 * (size_t)c.C * c.K, hipMemcpyDeviceToHost, c.stream), "emitted D2H");
+  HIPCHK(hipEventRecord(c.ev1, c.stream), "event");
+  HIPCHK(hipStreamSynchronize(c.stream), "source sync");
+  c.timing.ms_source += elapsed(c.ev0, c.ev1);
+  if (totalenergy) *totalenergy = c.h_sc.E_tot;
+  if (n_alive) *n_alive = c.h_sc.n_total;
+  return IMC_OK;
+}
+
+int imc_transport(imc_ctx* h, double* energydep, double* lostenergy, int64_t* segments) {
+  if (!h) return IMC_EINVAL;
+  Ctx& c = h->c;
+  if (!c.step_set) return fail(c, IMC_EINVAL, "imc_set_step_fields must be called before imc_transport");
+  int grid, tm; size_t lds;
+  transport_choose(c, &grid, &tm, &lds);
+  HIPCHK(hipEventRecord(c.ev0, c.stream), "event");
+  HIPCHK(launch_zero(c, (char*)c.d_sc + offsetof(Scalars, segments), sizeof(Scalars) - offsetof(Scalars, segments)),
+         "zero scalars");
+  if (tm == TM_GLOBAL) HIPCHK(launch_zero(c, c.d_dep, 8 * (size_t)c.NB), "zero tallies");
+  HIPCHK(hipEventRecord(c.evk0, c.stream), "event");
+  HIPCHK(launch_transport(c, grid, tm, lds), "transport kernel");
+  HIPCHK(hipEventRecord(c.evk1, c.stream), "event");
+  HIPCHK(launch_reduce_partials(c, grid, tm, c.NB, c.d_dep), "tally reduction");
+  if (energydep)
+    HIPCHK(hipMemcpyAsync(energydep, c.d_dep, 8 * (size_t)c.C * c.K, hipMemcpyDeviceToHost, c.stream), "energydep D2H");
+  double lost = 0.0;
+  HIPCHK(hipMemcpyAsync(&lost, c.d_dep + (c.NB - 1), 8, hipMemcpyDeviceToHost, c.stream), "lost D2H");
+  HIPCHK(hipMemcpyAsync(&c.h_sc, c.d_sc, sizeof(Scalars), hipMemcpyDeviceToHost, c.stream), "scalars D2H");
+  HIPCHK(hipEventRecord(c.ev1, c.stream), "event");
+  HIPCHK(hipStreamSynchronize(c.stream), "transport sync");
+  c.timing.ms_transport += elapsed(c.ev0, c.ev1);
+  c.timing.ms_transport_kernel += elapsed(c.evk0, c.evk1);
+  c.timing.n_transport_launches += 1;
+  c.timing.histories += c.h_sc.n_total;
+  c.timing.census_out += (int64_t)c.h_sc.census_out;
+  c.timing.segments += (int64_t)c.h_sc.segments;
+  if (lostenergy) *lostenergy = lost;
+  if (segments) *segments = (int64_t)c.h_sc.segments;
+  if (!std::isfinite(lost)) return fail(c, IMC_ENUMERIC, "non-finite lost energy");
+  return IMC_OK;
+}
+
+int imc_clean(imc_ctx* h, int64_t* n_alive) {
+  if (!h) return IMC_EINVAL;
+  Ctx& c = h->c;
+  if (!c.step_set) return fail(c, IMC_EINVAL, "imc_set_step_fields must be called before imc_clean");
+  HIPCHK(hipEventRecord(c.ev0, c.stream), "event");
+  HIPCHK(launch_clean(c, c.h_sc.n_total), "clean kernels");
+  int rc = fetch_scalars(c);
+  if (rc) return rc;
+  HIPCHK(hipEventRecord(c.ev1, c.stream), "event");
+  c.timing.ms_clean += elapsed(c.ev0, c.ev1);
+  if (n_alive) *n_alive = c.h_sc.n_alive;
+  return IMC_OK;
+}
+
+int imc_tally_census(imc_ctx* h, double* radenergydens) {
+  if (!h) return IMC_EINVAL;
+  Ctx& c = h->c;
+  if (!radenergydens) return fail(c, IMC_EINVAL, "radenergydens required");
+  HIPCHK(hipEventRecord(c.ev0, c.stream), "event");
+  HIPCHK(hipMemcpyAsync(radenergydens, c.d_rad, 8 * (size_t)c.C, hipMemcpyDeviceToHost, c.stream), "rad D2H");
+  HIPCHK(hipEventRecord(c.ev1, c.stream), "event");
+  HIPCHK(hipStreamSynchronize(c.stream), "tally sync");
+  c.timing.ms_tally += elapsed(c.ev0, c.ev1);
+  return IMC_OK;
+}
+
+int imc_particle_count(imc_ctx* h, int64_t* n) {
+  if (!h || !n) return IMC_EINVAL;
+  *n = h->c.h_sc.n_total;
+  return IMC_OK;
+}
+
+int imc_export_particles(imc_ctx* h, double* rec, int64_t cap, int64_t* nout) {
+  if (!h) return IMC_EINVAL;
+  Ctx& c = h->c;
+  const int64_t n = c.h_sc.n_total;
+  if (nout) *nout = n;
+  if (!rec || cap < n) return fail(c, IMC_EINVAL, "record buffer too small");
+  if (n == 0) return IMC_OK;
+  const size_t T = tsize(c.prec), W = wsize(c.prec);
+  const SoA& s = c.buf[c.cur];
+  std::vector<uint32_t> ck(n);
+  std::vector<unsigned char> x(n * T), y(c.is2d ? n * T : 1), mu(n * T), t(n * T), E(n * W), E0(n * W);
+  HIPCHK(hipMemcpyAsync(ck.data(), s.cellk, 4 * n, hipMemcpyDeviceToHost, c.stream), "export");
+  HIPCHK(hipMemcpyAsync(x.data(), s.x, T * n, hipMemcpyDeviceToHost, c.stream), "export");
+  if (c.is2d) HIPCHK(hipMemcpyAsync(y.data(), s.y, T * n, hipMemcpyDeviceToHost, c.stream), "export");
+  HIPCHK(hipMemcpyAsync(mu.data(), s.mu, T * n, hipMemcpyDeviceToHost, c.stream), "export");
+  HIPCHK(hipMemcpyAsync(t.data(), s.t, T * n, hipMemcpyDeviceToHost, c.stream), "export");
+  HIPCHK(hipMemcpyAsync(E.data(), s.E, W * n, hipMemcpyDeviceToHost, c.stream), "export");
+  HIPCHK(hipMemcpyAsync(E0.data(), s.E0, W * n, hipMemcpyDeviceToHost, c.stream), "export");
+  HIPCHK(hipStreamSynchronize(c.stream), "export sync");
+  auto tv = [&](const std::vector<unsigned char>& v, int64_t i) -> double {
+    if (c.prec == 64) return ((const double*)v.data())[i];
+    if (c.prec == 32) return (double)((const float*)v.data())[i];
+    return (double)(float)((const _Float16*)v.data())[i];
+  };
+  auto wv = [&](const std::vector<unsigned char>& v, int64_t i) -> double {
+    if (c.prec == 64) return ((const double*)v.data())[i];
+    return (double)((const float*)v.data())[i];
+  };
+  for (int64_t i = 0; i < n; ++i) {
+    const bool dead = ck[i] == DEAD;
+    const int cell = dead ? 0 : (int)(ck[i] & CELLMASK);
+    const int k = dead ? 0 : (int)(ck[i] >> KSHIFT);
+    const double scale = c.cfg.energyscales[k];
+    if (!c.is2d) {
+      double* r = rec + 9 * i;
+      r[0] = 0.0; r[1] = tv(t, i); r[2] = dead ? 0.0 : cell + 1; r[3] = tv(x, i); r[4] = tv(mu, i);
+      r[5] = 1.0; r[6] = wv(E, i); r[7] = dead ? -1.0 : wv(E0, i); r[8] = scale;
+    } else {
+      double* r = rec + 10 * i;
+      r[0] = tv(t, i); r[1] = dead ? 0.0 : cell % c.nx + 1; r[2] = dead ? 0.0 : cell / c.nx + 1;
+      r[3] = tv(x, i); r[4] = tv(y, i); r[5] = tv(mu, i); r[6] = 1.0;
+      r[7] = dead ? -1.0 : wv(E, i); r[8] = wv(E0, i); r[9] = scale;
+    }
+  }
+  return IMC_OK;
+}
+
+int imc_import_particles(imc_ctx* h, const double* rec, int64_t n) {
+  if (!h) return IMC_EINVAL;
+  Ctx& c = h->c;
+  if (n < 0 || (n > 0 && !rec)) return fail(c, IMC_EINVAL, "bad record buffer");
+  int rc = grow(c, n, 0);
+  if (rc) return rc;
+  const size_t T = tsize(c.prec), W = wsize(c.prec);
+  std::vector<uint32_t> ck(n);
+  std::vector<unsigned char> x(n * T + 1), y(n * T + 1), mu(n * T + 1), t(n * T + 1), E(n * W + 1), E0(n * W + 1);
+  auto put_t = [&](std::vector<unsigned char>& v, int64_t i, double val) {
+    if (c.prec == 64) ((double*)v.data())[i] = val;
+    else if (c.prec == 32) ((float*)v.data())[i] = (float)val;
+    else ((_Float16*)v.data())[i] = (_Float16)(float)val;
+  };
+  auto put_w = [&](std::vector<unsigned char>& v, int64_t i, double val) {
+    if (c.prec == 64) ((double*)v.data())[i] = val;
+    else ((float*)v.data())[i] = (float)val;
+  };
+  for (int64_t i = 0; i < n; ++i) {
+    const double* r = rec + (c.is2d ? 10 : 9) * i;
+    const double scale = c.is2d ? r[9] : r[8];
+    int k = 0;
+    for (int q = 0; q < c.K; ++q)
+      if (c.cfg.energyscales[q] == scale) { k = q; break; }
+    if (!c.is2d) {
+      const int cell = (int)r[2] - 1;
+      if (r[7] != -1.0 && (cell < 0 || cell >= c.C)) return fail(c, IMC_EINVAL, "record cell out of range");
+      ck[i] = r[7] == -1.0 ? DEAD : ((uint32_t)cell | ((uint32_t)k << KSHIFT));
+      put_t(t, i, r[1]); put_t(x, i, r[3]); put_t(mu, i, r[4]); put_w(E, i, r[6]); put_w(E0, i, r[7]);
+    } else {
+      const int ix = (int)r[1] - 1, iy = (int)r[2] - 1;
+      if (r[7] != -1.0 && (ix < 0 || ix >= c.nx || iy < 0 || iy >= c.ny))
+        return fail(c, IMC_EINVAL, "record cell out of range");
+      ck[i] = r[7] == -1.0 ? DEAD : ((uint32_t)(ix + c.nx * iy) | ((uint32_t)k << KSHIFT));
+      put_t(t, i, r[0]); put_t(x, i, r[3]); put_t(y, i, r[4]); put_t(mu, i, r[5]);
+      put_w(E, i, r[7]); put_w(E0, i, r[8]);
+    }
+  }
+  const SoA& s = c.buf[c.cur];
+  if (n > 0) {
+    HIPCHK(hipMemcpyAsync(s.cellk, ck.data(), 4 * n, hipMemcpyHostToDevice, c.stream), "import");
+    HIPCHK(hipMemcpyAsync(s.x, x.data(), T * n, hipMemcpyHostToDevice, c.stream), "import");
+    if (c.is2d) HIPCHK(hipMemcpyAsync(s.y, y.data(), T * n, hipMemcpyHostToDevice, c.stream), "import");
+    HIPCHK(hipMemcpyAsync(s.mu, mu.data(), T * n, hipMemcpyHostToDevice, c.stream), "import");
+    HIPCHK(hipMemcpyAsync(s.t, t.data(), T * n, hipMemcpyHostToDevice, c.stream), "import");
+    HIPCHK(hipMemcpyAsync(s.E, E.data(), W * n, hipMemcpyHostToDevice, c.stream), "import");
+    HIPCHK(hipMemcpyAsync(s.E0, E0.data(), W * n, hipMemcpyHostToDevice, c.stream), "import");
+  }
+  c.h_sc.n_census = n; c.h_sc.n_births = 0; c.h_sc.n_total = n; c.h_sc.n_alive = n;
+  HIPCHK(hipMemcpyAsync(c.d_sc, &c.h_sc, sizeof(Scalars), hipMemcpyHostToDevice, c.stream), "import scalars");
+  HIPCHK(hipStreamSynchronize(c.stream), "import sync");
+  return IMC_OK;
+}
+
+int imc_get_timing(imc_ctx* h, imc_timing* out) {
+  if (!h || !out) return IMC_EINVAL;
+  *out = h->c.timing;
+  return IMC_OK;
+}
+
+int imc_reset_timing(imc_ctx* h) {
+  if (!h) return IMC_EINVAL;
+  std::memset(&h->c.timing, 0, sizeof(imc_timing));
+  return IMC_OK;
+}
+
+int imc_synchronize(imc_ctx* h) {
+  if (!h) return IMC_EINVAL;
+  Ctx& c = h->c;
+  HIPCHK(hipStreamSynchronize(c.stream), "synchronize");
+  return IMC_OK;
+}
+
+int imc_state_bytes(const imc_ctx* h, int32_t* bytes) {
+  if (!h || !bytes) return IMC_EINVAL;
+  *bytes = state_bytes(h->c.prec, h->c.is2d);
+  return IMC_OK;
+}
+
+}  // extern "C"

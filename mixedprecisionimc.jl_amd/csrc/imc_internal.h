@@ -1,0 +1,120 @@
+// imc_internal.h -- library-private context and kernel-launcher declarations.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <string>
+
+#include "../../include/imc_hip.h"
+
+namespace imc {
+
+// Structure-of-arrays particle store in HBM.  Element types follow the precision:
+// x/y/mu(theta)/t in T, E/E0 in W, cellk = linear cell | scale index << 28 (DEAD if killed).
+struct SoA {
+  uint32_t* cellk = nullptr;
+  void* x = nullptr;
+  void* y = nullptr;      // 2D only
+  void* mu = nullptr;     // 1D: direction cosine; 2D: angle theta
+  void* t = nullptr;
+  void* E = nullptr;
+  void* E0 = nullptr;
+  int64_t cap = 0;
+};
+
+// Device-resident scalars of one step.
+struct Scalars {
+  int64_t n_census;            // particles in [0, n_census) before sourcing
+  int64_t n_births;            // births appended by the last sourcing
+  int64_t n_total;             // transport range [0, n_total)
+  int64_t n_alive;             // after clean
+  double E_tot;                // mesh.totalenergy
+  unsigned long long segments; // event-loop passes of the last transport
+  unsigned long long census_out;
+  uint32_t work;               // transport work-queue cursor
+  uint32_t pad;
+};
+
+struct Geo {                   // kernel view of the geometry and step coefficients
+  int nx, ny, C, K, NB;
+  int bc[4];
+  uint32_t seed, step;
+  double dt, c, s;             // dt, phys_c, distancescale
+  const void* dxs;             // T: dx*s (1D: per cell; 2D: per column)
+  const void* dys;             // T: dy*s per row (2D)
+  const void* dxw;             // W: dx
+  const void* dyw;             // W: dy
+  const void* sigcol;          // T: sigma_a(1-f)+sigma_s per cell
+  const void* sigdep;          // W: sigma_a*f per cell
+  double scales[16];
+};
+
+struct Ctx {
+  imc_config cfg;
+  int prec = 64, nx = 1, ny = 1, C = 1, K = 1, NB = 2;
+  bool is2d = false, geo_set = false, step_set = false;
+  std::string err;
+  hipStream_t stream = nullptr;
+  int device = 0, num_cu = 256;
+  int64_t step = 0;
+  double dt = 0.0;
+
+  // geometry / coefficients
+  double* d_dx = nullptr; double* d_dy = nullptr;          // FP64 copies
+  void* d_dxs = nullptr; void* d_dys = nullptr;            // T
+  void* d_dxw = nullptr; void* d_dyw = nullptr;            // W
+  double* d_temp = nullptr; double* d_fleck = nullptr; double* d_sa = nullptr; double* d_ss = nullptr;
+  double* d_radsrc = nullptr; double* d_tsurf = nullptr;   // tsurf packed (1D: 2, 2D: 2nx+2ny)
+  void* d_sigcol = nullptr; void* d_sigdep = nullptr;
+
+  // sourcing entries (faces, body cells, radiation-source cells)
+  int64_t NE = 0;
+  double* d_ent_e = nullptr;       // energy (W value)
+  double* d_ent_esc = nullptr;     // energy scale value
+  double* d_ent_term = nullptr;    // e/escale (W) as FP64
+  int32_t* d_ent_k = nullptr;      // scale index
+  int64_t* d_ent_cnt = nullptr;    // births
+  int64_t* d_ent_off = nullptr;    // exclusive prefix of births
+  double* d_emitted = nullptr;     // C x K
+  double* d_red = nullptr;         // tree-reduction scratch
+  int64_t red_len = 0;
+
+  // particles (ping-pong)
+  SoA buf[2];
+  int cur = 0;
+  Scalars* d_sc = nullptr;
+  Scalars h_sc{};                  // last copy of the scalars
+
+  // tallies
+  void* d_partial = nullptr;       // per-block partial sums
+  size_t partial_bytes = 0;
+  double* d_dep = nullptr;         // NB = C*K + 1 (last bin: lost energy)
+  double* d_rad = nullptr;         // C
+  int64_t* d_tile = nullptr;       // clean: per-tile counts/offsets
+  int64_t tile_len = 0;
+
+  // timing
+  hipEvent_t ev0 = nullptr, ev1 = nullptr, evk0 = nullptr, evk1 = nullptr;
+  imc_timing timing{};
+};
+
+Geo make_geo(const Ctx& c);
+int state_bytes(int prec, bool is2d);
+
+// Launchers (imc_kernels.hip).  Each returns a hipError_t.
+hipError_t launch_prep(Ctx& c);
+hipError_t launch_source_entries(Ctx& c);
+hipError_t launch_tree_sum(Ctx& c, const double* in, int64_t n, double* out_scalar);
+hipError_t launch_counts_and_scan(Ctx& c, int64_t n_input, int64_t n_max, double cellmin);
+hipError_t launch_births(Ctx& c, int64_t n_births);
+hipError_t launch_transport(Ctx& c, int grid_blocks, int tally_mode, size_t lds_bytes);
+hipError_t launch_clean(Ctx& c, int64_t n_total);
+hipError_t launch_reduce_partials(Ctx& c, int nblocks, int tally_mode, int nbins, double* out);
+hipError_t launch_zero(Ctx& c, void* p, size_t bytes);
+int transport_choose(const Ctx& c, int* grid_blocks, int* tally_mode, size_t* lds_bytes);
+int clean_choose(const Ctx& c, int* grid_blocks, int* tally_mode, size_t* lds_bytes);
+
+}  // namespace imc
+
+struct imc_ctx {
+  imc::Ctx c;
+};

@@ -1,0 +1,879 @@
+// imc_kernels.hip -- gfx950 kernels of the IMC particle engine.
+//
+//   k_prep            per-cell tracking coefficients (FP64 -> tracking precision)
+//   k_source_entries  emission / radiation-source / surface energies via the sorter
+//   k_tree256         deterministic FP64 tree sum (mesh.totalenergy)
+//   k_counts          particle budget and per-entry birth counts
+//   k_scan_counts     exclusive scan of birth counts (single block)
+//   k_births          one thread per birth: position, direction, time, weight
+//   k_transport       persistent event loop, one lane per history with lane refill,
+//                     wave-reduced tallies
+//   k_reduce_partials per-bin reduction of the per-block tally partials
+//   k_clean_*         order-preserving compaction fused with the census tally
+#include <hip/hip_runtime.h>
+
+#include "imc_device.h"
+#include "imc_internal.h"
+
+namespace imc {
+
+constexpr int TPB = 256;           // threads per block of the persistent kernels
+constexpr int NWAVES = TPB / 64;
+constexpr uint32_t CHUNK = 128;    // slots a wave claims from the work queue at a time
+constexpr int CLEAN_ITEMS = 8;     // slots per thread per clean tile
+constexpr int CLEAN_TILE = TPB * CLEAN_ITEMS;
+
+// ---------------------------------------------------------------------------------
+__global__ void k_fill_u8(unsigned char* p, size_t n) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0;
+}
+__global__ void k_fill_u64(unsigned long long* p, size_t n) {
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+    p[i] = 0ull;
+}
+
+hipError_t launch_zero(Ctx& c, void* p, size_t bytes) {
+  if (bytes == 0) return hipSuccess;
+  if ((bytes & 7) == 0 && (((uintptr_t)p) & 7) == 0) {
+    const size_t n = bytes / 8;
+    const int grid = (int)std::min<size_t>((n + 255) / 256, 4096);
+    k_fill_u64<<<grid, 256, 0, c.stream>>>((unsigned long long*)p, n);
+  } else {
+    k_fill_u8<<<(unsigned)((bytes + 255) / 256), 256, 0, c.stream>>>((unsigned char*)p, bytes);
+  }
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------------
+// Per-cell coefficients: FP64 formulas rounded once to the working types.
+// ---------------------------------------------------------------------------------
+template <int P>
+__global__ void k_prep(int nx, int ny, int C, double s, const double* __restrict__ dx,
+                       const double* __restrict__ dy, const double* __restrict__ sa,
+                       const double* __restrict__ ss, const double* __restrict__ f,
+                       typename Prec<P>::T* dxs, typename Prec<P>::T* dys,
+                       typename Prec<P>::W* dxw, typename Prec<P>::W* dyw,
+                       typename Prec<P>::T* sigcol, typename Prec<P>::W* sigdep) {
+#pragma clang fp contract(off)
+  using T = typename Prec<P>::T;
+  using W = typename Prec<P>::W;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < C) {
+    sigcol[i] = Tm<T>::from(sa[i] * (1.0 - f[i]) + ss[i]);   // imc_transport.jl:87
+    sigdep[i] = (W)(sa[i] * f[i]);                            // imc_transport.jl:95
+  }
+  if (i < nx) { dxs[i] = Tm<T>::from(dx[i] * s); dxw[i] = (W)dx[i]; }
+  if (i < ny) { dys[i] = Tm<T>::from(dy[i] * s); dyw[i] = (W)dy[i]; }
+}
+
+hipError_t launch_prep(Ctx& c) {
+  const int n = std::max(c.C, std::max(c.nx, c.ny));
+  const int grid = (n + 255) / 256;
+  const double s = c.cfg.distancescale;
+#define PREP(P)                                                                                  \
+  k_prep<P><<<grid, 256, 0, c.stream>>>(                                                        \
+      c.nx, c.ny, c.C, s, c.d_dx, c.d_dy, c.d_sa, c.d_ss, c.d_fleck,                             \
+      (Prec<P>::T*)c.d_dxs, (Prec<P>::T*)c.d_dys, (Prec<P>::W*)c.d_dxw, (Prec<P>::W*)c.d_dyw,     \
+      (Prec<P>::T*)c.d_sigcol, (Prec<P>::W*)c.d_sigdep)
+  if (c.prec == 64) PREP(64);
+  else if (c.prec == 32) PREP(32);
+  else PREP(16);
+#undef PREP
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------------
+// Sourcing energies (imc_sourcing.jl:56-111).  Entry order is the birth order of the
+// reference: 1D [left face, right face, body cells, source cells]; 2D [bottom faces,
+// top faces, left faces, right faces, body cells, source cells].
+// ---------------------------------------------------------------------------------
+struct SrcArgs {
+  int nx, ny, C, K;
+  int64_t NE;
+  double a, c, dt, s;
+  const double* temp; const double* fleck; const double* sa; const double* radsrc;
+  const double* tsurf; const double* dx; const double* dy;
+  double* ent_e; double* ent_esc; double* ent_term; int32_t* ent_k; double* emitted;
+  double scales[16];
+};
+
+template <int P, bool TWO_D>
+__global__ void k_source_entries(SrcArgs A) {
+#pragma clang fp contract(off)
+  using W = typename Prec<P>::W;
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= A.NE) return;
+  const double a = (double)(W)A.a, c = (double)(W)A.c, dt = (double)(W)A.dt, s = (double)(W)A.s;
+  const int nf = TWO_D ? 2 * A.nx + 2 * A.ny : 2;
+  W en = (W)0; double esc = 0.0; int k = 0;
+  if (e < nf) {
+    const double Tw = (double)(W)A.tsurf[e];
+    if (!TWO_D) {                                            // imc_sourcing.jl:60-61
+      const double v[8] = {a, c, Tw, Tw, Tw, Tw, dt, 0.25};
+      k = sorter_dev<W, 8>(v, true, A.scales, A.K, en, esc);
+    } else {                                                 // imc_sourcing.jl:86-93
+      const int ei = (int)e;
+      const double width = ei < 2 * A.nx ? A.dx[ei % A.nx] : A.dy[(ei - 2 * A.nx) % A.ny];
+      const double v[9] = {a, c, Tw, Tw, Tw, Tw, (double)(W)width, dt, 0.25};
+      k = sorter_dev<W, 9>(v, true, A.scales, A.K, en, esc);
+    }
+  } else if (e < nf + A.C) {                                 // body (+ emitted energy)
+    const int cell = (int)(e - nf);
+    const int ix = cell % A.nx, iy = cell / A.nx;
+    const double f = (double)(W)A.fleck[cell], sa = (double)(W)A.sa[cell], Tm_ = (double)(W)A.temp[cell];
+    const double dxw = (double)(W)A.dx[ix];
+    if (!TWO_D) {                                            // imc_sourcing.jl:67
+      const double v[11] = {f, sa, a, c, Tm_, Tm_, Tm_, Tm_, dxw, dt, s};
+      k = sorter_dev<W, 11>(v, false, A.scales, A.K, en, esc);
+    } else {                                                 // imc_sourcing.jl:102
+      const double v[12] = {f, sa, a, c, Tm_, Tm_, Tm_, Tm_, dxw, (double)(W)A.dy[iy], dt, s};
+      k = sorter_dev<W, 12>(v, false, A.scales, A.K, en, esc);
+    }
+    // emitted energy per volume (imc_sourcing.jl:69-70 / :104-105)
+    const double v2[10] = {f, sa, a, c, Tm_, Tm_, Tm_, Tm_, dt, s};
+    W em; double esc2;
+    const int k2 = sorter_dev<W, 10>(v2, false, A.scales, A.K, em, esc2);
+    for (int kk = 0; kk < A.K; ++kk) A.emitted[(size_t)kk * A.C + cell] = 0.0;
+    if (k2 > 0) A.emitted[(size_t)(k2 - 1) * A.C + cell] = (double)em;
+  } else {                                                   // radiation source
+    const int cell = (int)(e - nf - A.C);
+    const int ix = cell % A.nx, iy = cell / A.nx;
+    const double S = (double)(W)A.radsrc[cell], dxw = (double)(W)A.dx[ix];
+    if (!TWO_D) {                                            // imc_sourcing.jl:68
+      const double v[3] = {S, dxw, dt};
+      k = sorter_dev<W, 3>(v, false, A.scales, A.K, en, esc);
+    } else {                                                 // imc_sourcing.jl:103
+      const double v[4] = {S, dxw, (double)(W)A.dy[iy], dt};
+      k = sorter_dev<W, 4>(v, false, A.scales, A.K, en, esc);
+    }
+  }
+  A.ent_e[e] = (double)en;
+  A.ent_esc[e] = esc;
+  A.ent_k[e] = k > 0 ? k - 1 : 0;
+  A.ent_term[e] = (double)(en / (W)esc);
+}
+
+hipError_t launch_source_entries(Ctx& c) {
+  SrcArgs A;
+  A.nx = c.nx; A.ny = c.ny; A.C = c.C; A.K = c.K; A.NE = c.NE;
+  A.a = c.cfg.phys_a; A.c = c.cfg.phys_c; A.dt = c.dt; A.s = c.cfg.distancescale;
+  A.temp = c.d_temp; A.fleck = c.d_fleck; A.sa = c.d_sa; A.radsrc = c.d_radsrc; A.tsurf = c.d_tsurf;
+  A.dx = c.d_dx; A.dy = c.d_dy;
+  A.ent_e = c.d_ent_e; A.ent_esc = c.d_ent_esc; A.ent_term = c.d_ent_term; A.ent_k = c.d_ent_k;
+  A.emitted = c.d_emitted;
+  for (int k = 0; k < 16; ++k) A.scales[k] = c.cfg.energyscales[k];
+  const unsigned grid = (unsigned)((c.NE + 255) / 256);
+#define SRC(P, D) k_source_entries<P, D><<<grid, 256, 0, c.stream>>>(A)
+  if (c.prec == 64) { if (c.is2d) SRC(64, true); else SRC(64, false); }
+  else if (c.prec == 32) { if (c.is2d) SRC(32, true); else SRC(32, false); }
+  else { if (c.is2d) SRC(16, true); else SRC(16, false); }
+#undef SRC
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------------
+// Deterministic FP64 sum: blocks of 256 values reduced by the halving tree
+// s[j] += s[j+h], h = 128..1; repeated over block sums until one value remains.
+// ---------------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_tree256(const double* __restrict__ in, int64_t n,
+                                                 double* __restrict__ out) {
+#pragma clang fp contract(off)
+  __shared__ double s[256];
+  const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  s[threadIdx.x] = i < n ? in[i] : 0.0;
+  __syncthreads();
+  for (int h = 128; h >= 1; h >>= 1) {
+    if ((int)threadIdx.x < h) s[threadIdx.x] += s[threadIdx.x + h];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[blockIdx.x] = s[0];
+}
+
+hipError_t launch_tree_sum(Ctx& c, const double* in, int64_t n, double* out_scalar) {
+  // scratch: two ping-pong areas inside d_red
+  const double* src = in;
+  int64_t len = n;
+  double* bufs[2] = {c.d_red, c.d_red + (c.red_len / 2)};
+  int which = 0;
+  if (len <= 0) {
+    return launch_zero(c, out_scalar, sizeof(double));
+  }
+  for (;;) {
+    const int64_t nb = (len + 255) / 256;
+    double* dst = (nb == 1) ? out_scalar : bufs[which];
+    k_tree256<<<(unsigned)nb, 256, 0, c.stream>>>(src, len, dst);
+    hipError_t err = hipGetLastError();
+    if (err != hipSuccess) return err;
+    if (nb == 1) break;
+    src = dst; len = nb; which ^= 1;
+  }
+  return hipSuccess;
+}
+
+// ---------------------------------------------------------------------------------
+// Budget and counts (imc_sourcing.jl:132-157, 240-263).  Counts use FP64:
+// x = (e/escale) * n_source / E_tot, n = round-half-even(x), floored at CELLMIN where
+// the reference applies it.  ent_e is overwritten by the per-particle energy e/n (W).
+// ---------------------------------------------------------------------------------
+template <int P, bool TWO_D>
+__global__ void k_counts(int nx, int ny, int C, int64_t NE, int quirks, int64_t n_input,
+                         int64_t n_max, double cellmin, const Scalars* __restrict__ sc,
+                         const double* __restrict__ ent_term, double* __restrict__ ent_e,
+                         int64_t* __restrict__ ent_cnt) {
+#pragma clang fp contract(off)
+  using W = typename Prec<P>::W;
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= NE) return;
+  const int64_t n_census = sc->n_census;
+  const double E_tot = sc->E_tot;
+  double n_source = (double)n_input;
+  const int ndim = TWO_D ? 2 : 1;
+  if (n_input + n_census > n_max) n_source = fmax(cellmin, (double)(n_max - n_census - ndim - 1));
+  const int nf = TWO_D ? 2 * nx + 2 * ny : 2;
+  const double en = ent_e[e];
+  const double x = ent_term[e] * n_source / E_tot;
+  double cnt;
+  if (e < nf) {
+    if (TWO_D) cnt = en > 0 ? fmax(rint(x), cellmin) : 0.0;
+    else cnt = en > 0 ? rint(x) : 0.0;
+  } else if (e < nf + C) {
+    cnt = fmax(rint(x), cellmin);
+  } else {
+    cnt = en > 0 ? fmax(rint(x), cellmin) : 0.0;
+  }
+  const int64_t n_div = cnt > 0 ? (int64_t)cnt : 0;
+  int64_t n_loop = n_div;
+  if (TWO_D && (quirks & IMC_QUIRK_Q1_RADSRC_NBODY) && e >= nf + C && n_div > 0) {
+    const int64_t eb = e - C;                                // body entry of the same cell
+    const double xb = ent_term[eb] * n_source / E_tot;
+    const double cb = fmax(rint(xb), cellmin);
+    n_loop = cb > 0 ? (int64_t)cb : 0;
+  }
+  ent_cnt[e] = n_loop;
+  ent_e[e] = n_div > 0 ? (double)((W)en / (W)(double)n_div) : 0.0;
+}
+
+// Single-block exclusive scan of the birth counts; sets n_births / n_total.
+__global__ __launch_bounds__(1024) void k_scan_counts(const int64_t* __restrict__ cnt, int64_t n,
+                                                      int64_t* __restrict__ off, Scalars* sc) {
+  __shared__ int64_t wtot[16];
+  __shared__ int64_t running;
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  if (tid == 0) running = 0;
+  __syncthreads();
+  for (int64_t base = 0; base < n; base += 1024) {
+    const int64_t i = base + tid;
+    const int64_t v = i < n ? cnt[i] : 0;
+    int64_t incl = v;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+      const int64_t o = __shfl_up(incl, d, 64);
+      if (lane >= d) incl += o;
+    }
+    if (lane == 63) wtot[w] = incl;
+    __syncthreads();
+    int64_t wpre = 0;
+    for (int j = 0; j < w; ++j) wpre += wtot[j];
+    const int64_t r0 = running;
+    if (i < n) off[i] = r0 + wpre + incl - v;
+    __syncthreads();
+    if (tid == 1023) running = r0 + wpre + incl;
+    __syncthreads();
+  }
+  if (tid == 0) {
+    sc->n_births = running;
+    sc->n_total = sc->n_census + running;
+  }
+}
+
+hipError_t launch_counts_and_scan(Ctx& c, int64_t n_input, int64_t n_max, double cellmin) {
+  const unsigned grid = (unsigned)((c.NE + 255) / 256);
+#define CNT(P, D)                                                                               \
+  k_counts<P, D><<<grid, 256, 0, c.stream>>>(c.nx, c.ny, c.C, c.NE, c.cfg.quirks, n_input,     \
+                                             n_max, cellmin, c.d_sc, c.d_ent_term, c.d_ent_e,    \
+                                             c.d_ent_cnt)
+  if (c.prec == 64) { if (c.is2d) CNT(64, true); else CNT(64, false); }
+  else if (c.prec == 32) { if (c.is2d) CNT(32, true); else CNT(32, false); }
+  else { if (c.is2d) CNT(16, true); else CNT(16, false); }
+#undef CNT
+  hipError_t err = hipGetLastError();
+  if (err != hipSuccess) return err;
+  k_scan_counts<<<1, 1024, 0, c.stream>>>(c.d_ent_cnt, c.NE, c.d_ent_off, c.d_sc);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------------
+// Births (imc_sourcing.jl:161-236 in 1D, :265-366 in 2D).  One thread per birth b;
+// its entry is found by binary search over the exclusive prefix of counts.  Uniforms:
+// Philox(b_lo, b_hi, TAG_BIRTH, group) keyed by (seed, step); draw j of a birth is
+// word j of the groups (53-bit pairs for FP64, 24-bit words otherwise); direction
+// redraws use draws 8, 9, ...
+// ---------------------------------------------------------------------------------
+struct BirthArgs {
+  int nx, ny, C, K, quirks;
+  int64_t NE, n_births;
+  uint32_t seed, step;
+  double dt, s;
+  const double* dx; const double* dy;
+  const double* ent_nrg; const double* ent_esc; const int32_t* ent_k; const int64_t* ent_off;
+  const Scalars* sc;
+  SoA out;
+};
+
+template <int P, bool TWO_D>
+__global__ __launch_bounds__(256) void k_births(BirthArgs A) {
+#pragma clang fp contract(off)
+  using T = typename Prec<P>::T;
+  using W = typename Prec<P>::W;
+  using Cc = typename Prec<P>::Cc;
+  const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= A.n_births) return;
+  // entry: last e with off[e] <= b
+  int64_t lo = 0, hi = A.NE - 1;
+  while (lo < hi) {
+    const int64_t mid = (lo + hi + 1) >> 1;
+    if (A.ent_off[mid] <= b) lo = mid; else hi = mid - 1;
+  }
+  const int64_t e = lo;
+  const uint64_t ub = (uint64_t)b;
+  const W nrg = (W)A.ent_nrg[e];
+  const int k = A.ent_k[e];
+  const Cc dtc = (Cc)A.dt, sc = (Cc)A.s;
+  const double sW = (double)(W)A.s;
+  const int64_t slot = A.sc->n_census + b;
+  T* X = (T*)A.out.x; T* MU = (T*)A.out.mu; T* TT = (T*)A.out.t;
+  W* EE = (W*)A.out.E; W* E0 = (W*)A.out.E0;
+  uint32_t cell;
+  if (!TWO_D) {
+    const int C = A.C;
+    T x, mu, t;
+    if (e < 2) {                                             // surface births :161-189
+      const bool left = (e == 0);
+      cell = left ? 0u : (uint32_t)(C - 1);
+      const double dxe = (double)(W)A.dx[cell];
+      x = Tm<T>::from((left ? 0.01 : 0.99) * dxe * sW);
+      Cc m = BirthU<Cc>::sqrt_(BirthU<Cc>::u(ub, 1, A.seed, A.step));
+      for (int at = 0; m == (Cc)0; ++at) m = BirthU<Cc>::sqrt_(BirthU<Cc>::u(ub, 8 + at, A.seed, A.step));
+      mu = Tm<T>::from(left ? (double)m : -(double)m);
+      t = Tm<T>::from((double)(dtc * BirthU<Cc>::u(ub, 2, A.seed, A.step)));
+    } else {                                                 // body :193-214, source :217-236
+      cell = (uint32_t)((e - 2) % C);
+      x = Tm<T>::from((double)((Cc)A.dx[cell] * BirthU<Cc>::u(ub, 0, A.seed, A.step) * sc));
+      Cc m = (Cc)1 - (Cc)2 * BirthU<Cc>::u(ub, 1, A.seed, A.step);
+      for (int at = 0; m == (Cc)0; ++at) m = (Cc)1 - (Cc)2 * BirthU<Cc>::u(ub, 8 + at, A.seed, A.step);
+      mu = Tm<T>::from((double)m);
+      t = Tm<T>::from((double)(dtc * BirthU<Cc>::u(ub, 2, A.seed, A.step)));
+    }
+    X[slot] = x; MU[slot] = mu; TT[slot] = t;
+  } else {
+    const int nx = A.nx, ny = A.ny;
+    const int ei = (int)e;
+    int ix, iy;
+    T x, y, th, t;
+    if (ei < 2 * nx) {                                       // bottom/top :265-293
+      const bool bot = ei < nx;
+      ix = ei % nx; iy = bot ? 0 : ny - 1;
+      t = Tm<T>::from((double)(dtc * BirthU<Cc>::u(ub, 0, A.seed, A.step)));
+      x = Tm<T>::from((double)((Cc)A.dx[ix] * BirthU<Cc>::u(ub, 1, A.seed, A.step) * sc));
+      y = Tm<T>::from((bot ? 0.001 : 0.999) * (double)(W)A.dy[iy] * sW);
+      th = Tm<T>::from((bot ? PI_D : -PI_D) * (double)BirthU<Cc>::u(ub, 2, A.seed, A.step));
+    } else if (ei < 2 * nx + 2 * ny) {                       // left/right :295-323
+      const bool lf = ei < 2 * nx + ny;
+      iy = (ei - 2 * nx) % ny; ix = lf ? 0 : nx - 1;
+      t = Tm<T>::from((double)(dtc * BirthU<Cc>::u(ub, 0, A.seed, A.step)));
+      x = Tm<T>::from((lf ? 0.001 : 0.999) * (double)(W)A.dx[ix] * sW);
+      const double wy = (A.quirks & IMC_QUIRK_Q2_LR_FACE_DX) ? A.dx[iy] : A.dy[iy];
+      y = Tm<T>::from((double)((Cc)wy * BirthU<Cc>::u(ub, 1, A.seed, A.step) * sc));
+      const double u2 = (double)BirthU<Cc>::u(ub, 2, A.seed, A.step);
+      th = Tm<T>::from(PI_D * (lf ? (0.5 - u2) : (0.5 + u2)));
+    } else {                                                 // body / source :326-366
+      const int cellc = (int)((e - (2 * nx + 2 * ny)) % A.C);
+      ix = cellc % nx; iy = cellc / nx;
+      x = Tm<T>::from((double)((Cc)A.dx[ix] * BirthU<Cc>::u(ub, 0, A.seed, A.step) * sc));
+      y = Tm<T>::from((double)((Cc)A.dy[iy] * BirthU<Cc>::u(ub, 1, A.seed, A.step) * sc));
+      th = Tm<T>::from(TWO_PI_D * (double)BirthU<Cc>::u(ub, 2, A.seed, A.step));
+      t = Tm<T>::from((double)(dtc * BirthU<Cc>::u(ub, 3, A.seed, A.step)));
+    }
+    cell = (uint32_t)(ix + nx * iy);
+    X[slot] = x; ((T*)A.out.y)[slot] = y; MU[slot] = th; TT[slot] = t;
+  }
+  A.out.cellk[slot] = cell | ((uint32_t)k << KSHIFT);
+  EE[slot] = nrg;
+  E0[slot] = nrg;
+}
+
+hipError_t launch_births(Ctx& c, int64_t n_births) {
+  if (n_births <= 0) return hipSuccess;
+  BirthArgs A;
+  A.nx = c.nx; A.ny = c.ny; A.C = c.C; A.K = c.K; A.quirks = c.cfg.quirks;
+  A.NE = c.NE; A.n_births = n_births;
+  A.seed = c.cfg.seed; A.step = (uint32_t)c.step;
+  A.dt = c.dt; A.s = c.cfg.distancescale;
+  A.dx = c.d_dx; A.dy = c.d_dy;
+  A.ent_nrg = c.d_ent_e; A.ent_esc = c.d_ent_esc; A.ent_k = c.d_ent_k; A.ent_off = c.d_ent_off;
+  A.sc = c.d_sc; A.out = c.buf[c.cur];
+  const unsigned grid = (unsigned)((n_births + 255) / 256);
+#define BIR(P, D) k_births<P, D><<<grid, 256, 0, c.stream>>>(A)
+  if (c.prec == 64) { if (c.is2d) BIR(64, true); else BIR(64, false); }
+  else if (c.prec == 32) { if (c.is2d) BIR(32, true); else BIR(32, false); }
+  else { if (c.is2d) BIR(16, true); else BIR(16, false); }
+#undef BIR
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------------
+// Tally flush helpers shared by transport and clean.
+// ---------------------------------------------------------------------------------
+template <class Acc, int TM>
+__device__ __forceinline__ void tally_init(Acc* lds, int NB) {
+  if (TM == TM_WAVEROW) {
+    for (int i = threadIdx.x; i < NWAVES * NB * 2; i += blockDim.x) lds[i] = (Acc)0;
+  }
+  __syncthreads();
+}
+
+template <class Acc, int TM>
+__device__ __forceinline__ void tally_flush(const Acc* lds, int NB, Acc* partial) {
+  if (TM != TM_WAVEROW) return;
+  __syncthreads();
+  Acc* out = partial + (size_t)blockIdx.x * NB * 2;
+  for (int j = threadIdx.x; j < NB; j += blockDim.x) {
+    Acc s = (Acc)0, cc = (Acc)0;
+#pragma unroll
+    for (int w = 0; w < NWAVES; ++w) {
+      neumaier(s, cc, lds[(size_t)w * NB * 2 + 2 * j]);
+      cc += lds[(size_t)w * NB * 2 + 2 * j + 1];
+    }
+    out[2 * j] = s;
+    out[2 * j + 1] = cc;
+  }
+}
+
+template <class Acc>
+__global__ void k_reduce_partials(const Acc* __restrict__ partial, int nblocks, int NB,
+                                  double* __restrict__ out) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= NB) return;
+  Acc s = (Acc)0, cc = (Acc)0;
+  for (int g = 0; g < nblocks; ++g) {
+    neumaier(s, cc, partial[(size_t)g * NB * 2 + 2 * j]);
+    cc += partial[(size_t)g * NB * 2 + 2 * j + 1];
+  }
+  out[j] = (double)s + (double)cc;
+}
+
+hipError_t launch_reduce_partials(Ctx& c, int nblocks, int tally_mode, int nbins, double* out) {
+  if (tally_mode != TM_WAVEROW) return hipSuccess;
+  const unsigned grid = (unsigned)((nbins + 255) / 256);
+  if (c.prec == 64)
+    k_reduce_partials<double><<<grid, 256, 0, c.stream>>>((const double*)c.d_partial, nblocks, nbins, out);
+  else
+    k_reduce_partials<float><<<grid, 256, 0, c.stream>>>((const float*)c.d_partial, nblocks, nbins, out);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------------
+// Transport (imc_transport.jl:46-197 in 1D, :515-720 in 2D).
+// Persistent blocks; each wave pulls CHUNK-slot ranges from a device work counter and
+// refills a lane as soon as its history ends.  History in slot p uses Philox counters
+// (p, event, TAG_EVENT, redraw) keyed by (seed, step).
+// ---------------------------------------------------------------------------------
+template <int P, int TM, bool TWO_D>
+__global__ __launch_bounds__(TPB) void k_transport(SoA s, Geo g, Scalars* sc, void* partial_v,
+                                                   double* gbins) {
+  using T = typename Prec<P>::T;
+  using W = typename Prec<P>::W;
+  using Acc = typename Prec<P>::Acc;
+  extern __shared__ __align__(16) unsigned char smem[];
+  Acc* lds = (Acc*)smem;
+  const int NB = g.NB, C = g.C, nx = g.nx, ny = g.ny;
+  tally_init<Acc, TM>(lds, NB);
+  Tally<Acc, TM> tal{lds, gbins, NB};
+
+  const T* __restrict__ dxs = (const T*)g.dxs;
+  const T* __restrict__ dys = (const T*)g.dys;
+  const W* __restrict__ dxw = (const W*)g.dxw;
+  const W* __restrict__ dyw = (const W*)g.dyw;
+  const T* __restrict__ sigcol = (const T*)g.sigcol;
+  const W* __restrict__ sigdep = (const W*)g.sigdep;
+  uint32_t* __restrict__ CK = s.cellk;
+  T* __restrict__ X = (T*)s.x;
+  T* __restrict__ Y = (T*)s.y;
+  T* __restrict__ MU = (T*)s.mu;
+  T* __restrict__ TT = (T*)s.t;
+  W* __restrict__ EE = (W*)s.E;
+  const W* __restrict__ E0 = (const W*)s.E0;
+
+  const uint32_t n = (uint32_t)sc->n_total;
+  const int lane = lane_id();
+  const uint64_t lt_mask = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+  const T zero = (T)0, one = (T)1, two = (T)2;
+  const T cT = Tm<T>::from(g.c), sT = Tm<T>::from(g.s), dtT = Tm<T>::from(g.dt);
+
+  bool has = false;
+  uint32_t slot = 0, ev = 0;
+  int ix = 0, iy = 0, k = 0;
+  T x = zero, y = zero, mu = zero, t = zero, ox = zero, oy = zero;
+  W E = (W)0, Emin = (W)0;
+  uint32_t cur = 0, cend = 0;
+  bool done = false;
+  unsigned long long segs = 0, cens = 0;
+
+  for (;;) {
+    // ---- lane refill from the wave's chunk ----
+    uint64_t need = __ballot(!has);
+    while (need != 0ull && !done) {
+      if (cur >= cend) {
+        uint32_t base = 0;
+        if (lane == 0) base = atomicAdd(&sc->work, CHUNK);
+        base = __shfl(base, 0, 64);
+        if (base >= n) { done = true; break; }
+        cur = base;
+        cend = min(base + CHUNK, n);
+      }
+      const uint32_t avail = cend - cur;
+      const uint32_t rank = (uint32_t)__popcll(need & lt_mask);
+      if (!has && rank < avail) {
+        slot = cur + rank;
+        const uint32_t ck = CK[slot];
+        const int cell = (int)(ck & CELLMASK);
+        k = (int)(ck >> KSHIFT);
+        if (TWO_D) { ix = cell % nx; iy = cell / nx; y = Y[slot]; }
+        else ix = cell;
+        x = X[slot]; mu = MU[slot]; t = TT[slot]; E = EE[slot];
+        Emin = (W)(0.01 * (double)E0[slot]);                 // :61 / :531
+        if (TWO_D) { ox = Tm<T>::cos_(mu); oy = Tm<T>::sin_(mu); }
+        ev = 0;
+        has = true;
+      }
+      cur += min((uint32_t)__popcll(need), avail);
+      need = __ballot(!has);
+    }
+    if (!__any(has)) break;
+
+    int bin = -1, lbin = -1;
+    Acc val = (Acc)0, lval = (Acc)0;
+    if (has) {
+      ++segs;
+      const u4 q = philox(slot, ev, TAG_EVENT, 0u, g.seed, g.step);
+      const int cell = TWO_D ? ix + nx * iy : ix;
+      T db, dbx = zero, dby = zero;
+      if (!TWO_D) {
+        db = (mu > zero) ? (dxs[ix] - x) / mu : Tm<T>::abs_(x / mu);          // :77-83
+      } else {
+        dbx = (ox > zero) ? Tm<T>::abs_((dxs[ix] - x) / ox) : Tm<T>::abs_(x / ox);   // :538-542
+        dby = (oy > zero) ? Tm<T>::abs_((dys[iy] - y) / oy) : Tm<T>::abs_(y / oy);   // :544-548
+        if (Tm<T>::nan_(dbx)) db = dby;                                       // :551-557
+        else if (Tm<T>::nan_(dby)) db = dbx;
+        else db = jmin(dbx, dby);
+      }
+      const T dc = Tm<T>::expvar(q) / sigcol[cell];                           // :87 / :561
+      const T dcen = cT * (dtT - t) * sT;                                     // :89 / :569
+      const T d = jmin(jmin(db, dc), dcen);                                   // :92 / :571
+      const W tau = sigdep[cell] * (W)d;
+      const W Enew = E * Tm<W>::exp_(-tau);                                   // :95 / :580
+      const W Evol = TWO_D ? (E / dxw[ix]) / dyw[iy] : E / dxw[ix];
+      bin = k * C + cell;
+      if (Enew <= Emin) {                                                     // :97-106 / :586-595
+        val = (Acc)Evol;
+        CK[slot] = DEAD;
+        has = false;
+      } else {
+        val = (Acc)(-Evol * Tm<W>::expm1_(-tau));                              // :120 / :607
+        E = Enew;
+        t = t + (d / sT) / cT;
+        bool dead = false, face = false;
+        if (!TWO_D) {
+          x = x + mu * d;                                                     // :124
+          if (d == db) {                                                      // :130-170
+            if (mu > zero) {
+              if (ix == C - 1) {
+                if (g.bc[1] == IMC_REFLECT) mu = -mu; else dead = true;
+              }
+              if (!dead) { ix += 1; x = zero; }
+            }
+            if (!dead && mu < zero) {
+              if (ix == 0) {
+                if (g.bc[0] == IMC_REFLECT) mu = -mu; else dead = true;
+              } else {
+                ix -= 1; x = dxs[ix];
+              }
+            }
+          }
+        } else {
+          x = x + d * ox;                                                     // :615-616
+          y = y + d * oy;
+          if (d == dbx || d == dby) {                                         // :621-704
+            face = true;
+            if (dbx < dby) {
+              if (ox > zero) {
+                if (ix == nx - 1) {
+                  if (g.bc[1] == IMC_REFLECT) mu = Tm<T>::atan2_(oy, -ox); else dead = true;
+                } else { ix += 1; x = zero; }
+              } else {
+                if (ix == 0) {
+                  if (g.bc[0] == IMC_REFLECT) mu = Tm<T>::atan2_(oy, -ox); else dead = true;
+                } else { ix -= 1; x = dxs[ix]; }
+              }
+            } else {
+              if (oy > zero) {
+                if (iy == ny - 1) {
+                  if (g.bc[2] == IMC_REFLECT) mu = Tm<T>::atan2_(-oy, ox); else dead = true;
+                } else { iy += 1; y = zero; }
+              } else {
+                if (iy == 0) {
+                  if (g.bc[3] == IMC_REFLECT) mu = Tm<T>::atan2_(-oy, ox); else dead = true;
+                } else { iy -= 1; y = dys[iy]; }
+              }
+            }
+            if (!dead) { ox = Tm<T>::cos_(mu); oy = Tm<T>::sin_(mu); }
+          }
+        }
+        if (dead) {
+          lbin = NB - 1;
+          lval = (Acc)(E / (W)g.scales[k]);
+          CK[slot] = DEAD;
+          has = false;
+        } else if (!face) {
+          if (d == dc) {                                                      // :174-183 / :706-710
+            if (!TWO_D) {
+              T m = one - two * Tm<T>::uscat(q, true);
+              for (uint32_t at = 1; m == zero; ++at)
+                m = one - two * Tm<T>::uscat(philox(slot, ev, TAG_EVENT, at, g.seed, g.step), false);
+              mu = m;
+            } else {
+              mu = Tm<T>::from(TWO_PI_D * Tm<T>::d(Tm<T>::uscat(q, true)));
+              ox = Tm<T>::cos_(mu); oy = Tm<T>::sin_(mu);
+            }
+          }
+          if (d == dcen) {                                                    // :185-193 / :712-717
+            const int cl = TWO_D ? ix + nx * iy : ix;
+            CK[slot] = (uint32_t)cl | ((uint32_t)k << KSHIFT);
+            X[slot] = x;
+            if (TWO_D) Y[slot] = y;
+            MU[slot] = mu;
+            TT[slot] = zero;
+            EE[slot] = E;
+            ++cens;
+            has = false;
+          }
+        }
+      }
+      ++ev;
+    }
+    tal.add(bin, val);
+    tal.add(lbin, lval);
+  }
+
+  segs = wave_sum(segs);
+  cens = wave_sum(cens);
+  if (lane == 0) {
+    atomicAdd(&sc->segments, segs);
+    atomicAdd(&sc->census_out, cens);
+  }
+  tally_flush<Acc, TM>(lds, NB, (Acc*)partial_v);
+}
+
+int transport_choose(const Ctx& c, int* grid_blocks, int* tally_mode, size_t* lds_bytes) {
+  const size_t acc = c.prec == 64 ? 8 : 4;
+  const size_t wave_rows = (size_t)NWAVES * c.NB * 2 * acc;
+  if (wave_rows <= 64 * 1024) { *tally_mode = TM_WAVEROW; *lds_bytes = wave_rows; }
+  else { *tally_mode = TM_GLOBAL; *lds_bytes = 0; }
+  int per_cu = 0;
+  hipError_t err;
+#define OCC(P, TM_, D)                                                                             \
+  err = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_transport<P, TM_, D>, TPB, *lds_bytes)
+#define OCCD(P, TM_) do { if (c.is2d) OCC(P, TM_, true); else OCC(P, TM_, false); } while (0)
+#define OCCT(P) do { if (*tally_mode == TM_WAVEROW) OCCD(P, TM_WAVEROW); else OCCD(P, TM_GLOBAL); } while (0)
+  if (c.prec == 64) OCCT(64);
+  else if (c.prec == 32) OCCT(32);
+  else OCCT(16);
+#undef OCCT
+#undef OCCD
+#undef OCC
+  if (err != hipSuccess || per_cu <= 0) per_cu = 1;
+  *grid_blocks = per_cu * c.num_cu;
+  return 0;
+}
+
+hipError_t launch_transport(Ctx& c, int grid, int tm, size_t lds) {
+  const Geo g = make_geo(c);
+  SoA s = c.buf[c.cur];
+#define TR(P, TM_, D)                                                                            \
+  k_transport<P, TM_, D><<<grid, TPB, lds, c.stream>>>(s, g, c.d_sc, c.d_partial, c.d_dep)
+#define TRD(P, TM_) do { if (c.is2d) TR(P, TM_, true); else TR(P, TM_, false); } while (0)
+#define TRT(P) do { if (tm == TM_WAVEROW) TRD(P, TM_WAVEROW); else TRD(P, TM_GLOBAL); } while (0)
+  if (c.prec == 64) TRT(64);
+  else if (c.prec == 32) TRT(32);
+  else TRT(16);
+#undef TRT
+#undef TRD
+#undef TR
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------------
+// Clean (imc_clean.jl:6-19) fused with the census tally (imc_tally.jl:157-190).
+//   k_clean_count  : alive count per tile of CLEAN_TILE slots
+//   k_clean_scan   : exclusive scan of tile counts (single block); n_alive
+//   k_clean_scatter: stable scatter of the live particles to the other buffer while
+//                    tallying E/(dx*scale) (1D) or E/(dx*dy*scale) (2D) per cell
+// ---------------------------------------------------------------------------------
+__global__ __launch_bounds__(TPB) void k_clean_count(const uint32_t* __restrict__ ck, int64_t n,
+                                                     int64_t* __restrict__ tile_cnt) {
+  __shared__ int wsum[NWAVES];
+  const int64_t base = (int64_t)blockIdx.x * CLEAN_TILE;
+  int cnt = 0;
+#pragma unroll
+  for (int j = 0; j < CLEAN_ITEMS; ++j) {
+    const int64_t i = base + j * TPB + threadIdx.x;
+    if (i < n && ck[i] != DEAD) ++cnt;
+  }
+  cnt = wave_sum(cnt);
+  if (lane_id() == 0) wsum[threadIdx.x >> 6] = cnt;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int t = 0;
+    for (int w = 0; w < NWAVES; ++w) t += wsum[w];
+    tile_cnt[blockIdx.x] = t;
+  }
+}
+
+__global__ __launch_bounds__(1024) void k_clean_scan(int64_t* __restrict__ tiles, int64_t ntiles,
+                                                     Scalars* sc) {
+  __shared__ int64_t wtot[16];
+  __shared__ int64_t running;
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  if (tid == 0) running = 0;
+  __syncthreads();
+  for (int64_t base = 0; base < ntiles; base += 1024) {
+    const int64_t i = base + tid;
+    const int64_t v = i < ntiles ? tiles[i] : 0;
+    int64_t incl = v;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+      const int64_t o = __shfl_up(incl, d, 64);
+      if (lane >= d) incl += o;
+    }
+    if (lane == 63) wtot[w] = incl;
+    __syncthreads();
+    int64_t wpre = 0;
+    for (int j = 0; j < w; ++j) wpre += wtot[j];
+    const int64_t r0 = running;
+    if (i < ntiles) tiles[i] = r0 + wpre + incl - v;
+    __syncthreads();
+    if (tid == 1023) running = r0 + wpre + incl;
+    __syncthreads();
+  }
+  if (tid == 0) {
+    sc->n_alive = running;
+    sc->n_census = running;
+    sc->n_births = 0;
+    sc->n_total = running;
+  }
+}
+
+template <int P, int TM, bool TWO_D>
+__global__ __launch_bounds__(TPB) void k_clean_scatter(SoA src, SoA dst, int64_t n, int64_t ntiles,
+                                                       const int64_t* __restrict__ tile_off, Geo g,
+                                                       void* partial_v, double* gbins) {
+  using T = typename Prec<P>::T;
+  using W = typename Prec<P>::W;
+  using Acc = typename Prec<P>::Acc;
+  extern __shared__ __align__(16) unsigned char smem[];
+  Acc* lds = (Acc*)smem;
+  __shared__ int wcnt[NWAVES];
+  const int NB = g.C;
+  tally_init<Acc, TM>(lds, NB);
+  Tally<Acc, TM> tal{lds, gbins, NB};
+  const W* __restrict__ dxw = (const W*)g.dxw;
+  const W* __restrict__ dyw = (const W*)g.dyw;
+  const int lane = lane_id(), w = threadIdx.x >> 6;
+  const uint64_t lt_mask = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+  for (int64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+    int64_t out = tile_off[tile];
+    const int64_t base = tile * CLEAN_TILE;
+    for (int j = 0; j < CLEAN_ITEMS; ++j) {
+      const int64_t i = base + j * TPB + threadIdx.x;
+      const uint32_t ck = i < n ? src.cellk[i] : DEAD;
+      const bool alive = ck != DEAD;
+      const uint64_t bal = __ballot(alive);
+      if (lane == 0) wcnt[w] = __popcll(bal);
+      __syncthreads();
+      int pre = 0, tot = 0;
+#pragma unroll
+      for (int q = 0; q < NWAVES; ++q) { pre += q < w ? wcnt[q] : 0; tot += wcnt[q]; }
+      int bin = -1;
+      Acc val = (Acc)0;
+      if (alive) {
+        const int64_t o = out + pre + __popcll(bal & lt_mask);
+        const W E = ((const W*)src.E)[i];
+        dst.cellk[o] = ck;
+        ((T*)dst.x)[o] = ((const T*)src.x)[i];
+        if (TWO_D) ((T*)dst.y)[o] = ((const T*)src.y)[i];
+        ((T*)dst.mu)[o] = ((const T*)src.mu)[i];
+        ((T*)dst.t)[o] = ((const T*)src.t)[i];
+        ((W*)dst.E)[o] = E;
+        ((W*)dst.E0)[o] = ((const W*)src.E0)[i];
+        const int cell = (int)(ck & CELLMASK);
+        const W sc = (W)g.scales[ck >> KSHIFT];
+        bin = cell;
+        if (TWO_D) val = (Acc)(E / ((dxw[cell % g.nx] * dyw[cell / g.nx]) * sc));
+        else val = (Acc)(E / (dxw[cell] * sc));
+      }
+      tal.add(bin, val);
+      out += tot;
+      __syncthreads();
+    }
+  }
+  tally_flush<Acc, TM>(lds, NB, (Acc*)partial_v);
+}
+
+int clean_choose(const Ctx& c, int* grid_blocks, int* tally_mode, size_t* lds_bytes) {
+  const size_t acc = c.prec == 64 ? 8 : 4;
+  const size_t wave_rows = (size_t)NWAVES * c.C * 2 * acc;
+  if (wave_rows <= 64 * 1024) { *tally_mode = TM_WAVEROW; *lds_bytes = wave_rows; }
+  else { *tally_mode = TM_GLOBAL; *lds_bytes = 0; }
+  *grid_blocks = 2 * c.num_cu;
+  return 0;
+}
+
+hipError_t launch_clean(Ctx& c, int64_t n_total) {
+  const int64_t ntiles = (n_total + CLEAN_TILE - 1) / CLEAN_TILE;
+  if (ntiles > 0) {
+    k_clean_count<<<(unsigned)ntiles, TPB, 0, c.stream>>>(c.buf[c.cur].cellk, n_total, c.d_tile);
+    hipError_t err = hipGetLastError();
+    if (err != hipSuccess) return err;
+  }
+  k_clean_scan<<<1, 1024, 0, c.stream>>>(c.d_tile, ntiles, c.d_sc);
+  hipError_t err = hipGetLastError();
+  if (err != hipSuccess) return err;
+  int grid, tm; size_t lds;
+  clean_choose(c, &grid, &tm, &lds);
+  if (tm == TM_GLOBAL) {
+    err = launch_zero(c, c.d_rad, sizeof(double) * c.C);
+    if (err != hipSuccess) return err;
+  }
+  const Geo g = make_geo(c);
+  SoA src = c.buf[c.cur], dst = c.buf[c.cur ^ 1];
+#define CL(P, TM_, D)                                                                                \
+  k_clean_scatter<P, TM_, D><<<grid, TPB, lds, c.stream>>>(src, dst, n_total, ntiles, c.d_tile, g, \
+                                                          c.d_partial, c.d_rad)
+#define CLD(P, TM_) do { if (c.is2d) CL(P, TM_, true); else CL(P, TM_, false); } while (0)
+#define CLT(P) do { if (tm == TM_WAVEROW) CLD(P, TM_WAVEROW); else CLD(P, TM_GLOBAL); } while (0)
+  if (c.prec == 64) CLT(64);
+  else if (c.prec == 32) CLT(32);
+  else CLT(16);
+#undef CLT
+#undef CLD
+#undef CL
+  err = hipGetLastError();
+  if (err != hipSuccess) return err;
+  err = launch_reduce_partials(c, grid, tm, c.C, c.d_rad);
+  if (err != hipSuccess) return err;
+  c.cur ^= 1;
+  return hipSuccess;
+}
+
+}  // namespace imc
