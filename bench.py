@@ -28,15 +28,40 @@ DECK = os.path.join(ROOT, "decks", "c2_marshak_fp32.txt")
 HBM_PEAK_GBS = 8000.0
 
 
+TALLY_MODES = {0: "WAVEROW", 1: "BLOCKROW", 2: "GLOBAL"}
+
+
+def gpu_window(inputs, steps: int):
+    """GPU rate on the same sample as the CPU baseline: the first `steps` timesteps."""
+    import MixedPrecisionIMC as M
+
+    mesh, simvars, particles = M.setup(inputs)
+    ts = inputs["TIMESTEPPING"].upper() if isinstance(inputs["TIMESTEPPING"], str) else "RAMP"
+    particles.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        M.step(inputs, mesh, simvars, particles, ts)
+    particles.synchronize()
+    wall = time.perf_counter() - t0
+    tm = particles.timing()
+    particles.close()
+    hot = (tm["ms_source"] + tm["ms_transport"] + tm["ms_clean"] + tm["ms_tally"]) / 1e3
+    return tm["histories"] / wall, tm["histories"] / hot
+
+
 def cpu_baseline(inputs, mesh, steps: int):
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O  # the checker, timed here as the CPU baseline only
 
     threads = min(16, os.cpu_count() or 1)
     r = O.run_deck(inputs, mesh, max_steps=steps, nthreads=threads)
-    return {"value": r["histories"] / r["t_hot"], "unit": "histories/s", "cores": threads, "kind": "port",
+    g_wall, g_hot = gpu_window(inputs, steps)
+    cpu = r["histories"] / r["t_hot"]
+    return {"value": cpu, "unit": "histories/s", "cores": threads, "kind": "port",
             "sample": f"first {r['steps']} timesteps of the same deck from t=0 ({r['histories']} histories, "
-                      f"C++ FP32 oracle, OpenMP over particles), source+transport+clean+census time"}
+                      f"C++ FP32 oracle, OpenMP over particles), source+transport+clean+census time",
+            "gpu_same_sample": {"value_wall": g_wall, "value_hot_path": g_hot,
+                                "speedup_hot_path": g_hot / cpu}}
 
 
 def main():
@@ -99,7 +124,8 @@ def main():
                    "state_bytes": sbytes},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "k_transport<32,WAVEROW,1D>",
+                     "kernel": f"k_transport<32,{TALLY_MODES.get(tm['transport_tally_mode'])},1D>",
+                     "grid_blocks": tm["transport_grid"],
                      "avg_launch_ms": tm["ms_transport_kernel"] / launches,
                      "bytes_per_launch": alg_bytes / launches},
         "gpu_time_ms": {k: tm[k] for k in ("ms_source", "ms_transport", "ms_clean", "ms_tally",

@@ -61,6 +61,8 @@ typedef struct {
   int64_t histories;              /* particles handed to transport */
   int64_t census_out;             /* particles written back to census */
   int64_t segments;               /* event-loop passes (the reference's `iterations`) */
+  int32_t transport_tally_mode;   /* 0 wave rows, 1 block row (LDS atomics), 2 global atomics */
+  int32_t transport_grid;         /* persistent transport blocks of the last launch */
 } imc_timing;
 
 /* Creates a context: MixedPrecisionIMC.jl:117 (particles = Vector{Vector{}}()) plus the

@@ -64,6 +64,8 @@ class ImcTiming(ctypes.Structure):
         ("histories", ctypes.c_int64),
         ("census_out", ctypes.c_int64),
         ("segments", ctypes.c_int64),
+        ("transport_tally_mode", ctypes.c_int32),
+        ("transport_grid", ctypes.c_int32),
     ]
 
 

@@ -351,6 +351,8 @@ int imc_transport(imc_ctx* h, double* energydep, double* lostenergy, int64_t* se
   c.timing.histories += c.h_sc.n_total;
   c.timing.census_out += (int64_t)c.h_sc.census_out;
   c.timing.segments += (int64_t)c.h_sc.segments;
+  c.timing.transport_tally_mode = tm;
+  c.timing.transport_grid = grid;
   if (lostenergy) *lostenergy = lost;
   if (segments) *segments = (int64_t)c.h_sc.segments;
   if (!std::isfinite(lost)) return fail(c, IMC_ENUMERIC, "non-finite lost energy");

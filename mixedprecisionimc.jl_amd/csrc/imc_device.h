@@ -217,12 +217,16 @@ __device__ __forceinline__ void neumaier(double& s, double& c, double v) {
   s = t;
 }
 
-// Tally storage modes.
-//  WAVEROW : every wave owns a private LDS row of (sum, comp) pairs per bin; the wave
-//            first reduces the lanes that hit the same bin (tree in the wave), then its
-//            leader lane applies a Neumaier update -- no atomics, compensated.
-//  BLOCKROW: one LDS row per block of Acc, updated by the wave leaders with LDS atomics.
-//  GLOBAL  : wave leaders add straight into the global FP64 bins with atomics.
+// Tally storage modes (all keep per-block partial sums that a deterministic compensated
+// reduction combines across blocks, except GLOBAL):
+//  BLOCKROW: one LDS row per block; every lane adds its deposit with an LDS atomic.
+//            Each block bin sums only that block's deposits (a few hundred terms), so FP32
+//            rounding stays near 1e-6 relative before the compensated cross-block sum.
+//  WAVEROW : every wave owns a private LDS row of (sum, comp) pairs; lanes hitting the
+//            same bin are first summed by a tree inside the wave, then the leader lane
+//            applies a Neumaier update -- no atomics, compensated, deterministic.
+//  GLOBAL  : lanes add straight into the global FP64 bins with atomics (meshes whose
+//            bins do not fit in LDS).
 enum TallyMode { TM_WAVEROW = 0, TM_BLOCKROW = 1, TM_GLOBAL = 2 };
 
 template <class Acc, int TM>
@@ -233,6 +237,14 @@ struct Tally {
 
   // All 64 lanes must call this together.  bin < 0 marks "no deposit".
   __device__ __forceinline__ void add(int bin, Acc val) {
+    if (TM == TM_BLOCKROW) {
+      if (bin >= 0) atomicAdd(&lds[bin], val);
+      return;
+    }
+    if (TM == TM_GLOBAL) {
+      if (bin >= 0) atomicAdd(&gbins[bin], (double)val);
+      return;
+    }
     const int lane = lane_id();
     uint64_t pend = __ballot(bin >= 0);
     while (pend) {
@@ -241,16 +253,10 @@ struct Tally {
       const bool mine = (bin == b);
       Acc v = wave_sum(mine ? val : (Acc)0);
       if (lane == leader) {
-        if (TM == TM_WAVEROW) {
-          Acc* row = lds + (size_t)(threadIdx.x >> 6) * NB * 2;
-          Acc s = row[2 * b], c = row[2 * b + 1];
-          neumaier(s, c, v);
-          row[2 * b] = s; row[2 * b + 1] = c;
-        } else if (TM == TM_BLOCKROW) {
-          atomicAdd(&lds[b], v);
-        } else {
-          atomicAdd(&gbins[b], (double)v);
-        }
+        Acc* row = lds + (size_t)(threadIdx.x >> 6) * NB * 2;
+        Acc s = row[2 * b], c = row[2 * b + 1];
+        neumaier(s, c, v);
+        row[2 * b] = s; row[2 * b + 1] = c;
       }
       pend &= ~__ballot(mine);
     }

@@ -429,47 +429,70 @@ template <class Acc, int TM>
 __device__ __forceinline__ void tally_init(Acc* lds, int NB) {
   if (TM == TM_WAVEROW) {
     for (int i = threadIdx.x; i < NWAVES * NB * 2; i += blockDim.x) lds[i] = (Acc)0;
+  } else if (TM == TM_BLOCKROW) {
+    for (int i = threadIdx.x; i < NB; i += blockDim.x) lds[i] = (Acc)0;
   }
   __syncthreads();
 }
 
+// Writes this block's (sum, compensation) per bin to partial[blockIdx.x][bin][2].
 template <class Acc, int TM>
 __device__ __forceinline__ void tally_flush(const Acc* lds, int NB, Acc* partial) {
-  if (TM != TM_WAVEROW) return;
+  if (TM == TM_GLOBAL) return;
   __syncthreads();
   Acc* out = partial + (size_t)blockIdx.x * NB * 2;
   for (int j = threadIdx.x; j < NB; j += blockDim.x) {
     Acc s = (Acc)0, cc = (Acc)0;
+    if (TM == TM_WAVEROW) {
 #pragma unroll
-    for (int w = 0; w < NWAVES; ++w) {
-      neumaier(s, cc, lds[(size_t)w * NB * 2 + 2 * j]);
-      cc += lds[(size_t)w * NB * 2 + 2 * j + 1];
+      for (int w = 0; w < NWAVES; ++w) {
+        neumaier(s, cc, lds[(size_t)w * NB * 2 + 2 * j]);
+        cc += lds[(size_t)w * NB * 2 + 2 * j + 1];
+      }
+    } else {
+      s = lds[j];
     }
     out[2 * j] = s;
     out[2 * j + 1] = cc;
   }
 }
 
+// Cross-block reduction, one block per bin: each thread folds a strided subset of the
+// block partials with Neumaier compensation, then a fixed halving tree merges the
+// threads' (sum, comp) pairs.  The order is fixed, so the result is deterministic for
+// given partials.
 template <class Acc>
-__global__ void k_reduce_partials(const Acc* __restrict__ partial, int nblocks, int NB,
-                                  double* __restrict__ out) {
-  const int j = blockIdx.x * blockDim.x + threadIdx.x;
-  if (j >= NB) return;
+__global__ __launch_bounds__(256) void k_reduce_partials(const Acc* __restrict__ partial, int nblocks,
+                                                          int NB, double* __restrict__ out) {
+  __shared__ Acc ss[256], sc[256];
+  const int j = blockIdx.x;
   Acc s = (Acc)0, cc = (Acc)0;
-  for (int g = 0; g < nblocks; ++g) {
+  for (int g = threadIdx.x; g < nblocks; g += 256) {
     neumaier(s, cc, partial[(size_t)g * NB * 2 + 2 * j]);
     cc += partial[(size_t)g * NB * 2 + 2 * j + 1];
   }
-  out[j] = (double)s + (double)cc;
+  ss[threadIdx.x] = s;
+  sc[threadIdx.x] = cc;
+  __syncthreads();
+  for (int h = 128; h >= 1; h >>= 1) {
+    if ((int)threadIdx.x < h) {
+      Acc a = ss[threadIdx.x], ac = sc[threadIdx.x];
+      neumaier(a, ac, ss[threadIdx.x + h]);
+      ac += sc[threadIdx.x + h];
+      ss[threadIdx.x] = a;
+      sc[threadIdx.x] = ac;
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[j] = (double)ss[0] + (double)sc[0];
 }
 
 hipError_t launch_reduce_partials(Ctx& c, int nblocks, int tally_mode, int nbins, double* out) {
-  if (tally_mode != TM_WAVEROW) return hipSuccess;
-  const unsigned grid = (unsigned)((nbins + 255) / 256);
+  if (tally_mode == TM_GLOBAL) return hipSuccess;
   if (c.prec == 64)
-    k_reduce_partials<double><<<grid, 256, 0, c.stream>>>((const double*)c.d_partial, nblocks, nbins, out);
+    k_reduce_partials<double><<<nbins, 256, 0, c.stream>>>((const double*)c.d_partial, nblocks, nbins, out);
   else
-    k_reduce_partials<float><<<grid, 256, 0, c.stream>>>((const float*)c.d_partial, nblocks, nbins, out);
+    k_reduce_partials<float><<<nbins, 256, 0, c.stream>>>((const float*)c.d_partial, nblocks, nbins, out);
   return hipGetLastError();
 }
 
@@ -675,17 +698,30 @@ __global__ __launch_bounds__(TPB) void k_transport(SoA s, Geo g, Scalars* sc, vo
   tally_flush<Acc, TM>(lds, NB, (Acc*)partial_v);
 }
 
-int transport_choose(const Ctx& c, int* grid_blocks, int* tally_mode, size_t* lds_bytes) {
+// Tally mode: cfg.reserved[0] (0 auto, 1 WAVEROW, 2 BLOCKROW, 3 GLOBAL).  Auto keeps a
+// block row in LDS when it fits in 64 KiB and otherwise adds into global bins.
+static void pick_tally(const Ctx& c, int nbins, int* tally_mode, size_t* lds_bytes) {
   const size_t acc = c.prec == 64 ? 8 : 4;
-  const size_t wave_rows = (size_t)NWAVES * c.NB * 2 * acc;
-  if (wave_rows <= 64 * 1024) { *tally_mode = TM_WAVEROW; *lds_bytes = wave_rows; }
+  const size_t wave_rows = (size_t)NWAVES * nbins * 2 * acc, block_row = (size_t)nbins * acc;
+  int req = c.cfg.reserved[0];
+  if (req == 1 && wave_rows > 64 * 1024) req = 0;
+  if (req == 2 && block_row > 64 * 1024) req = 0;
+  if (req == 1) { *tally_mode = TM_WAVEROW; *lds_bytes = wave_rows; }
+  else if (req == 3) { *tally_mode = TM_GLOBAL; *lds_bytes = 0; }
+  else if (block_row <= 64 * 1024) { *tally_mode = TM_BLOCKROW; *lds_bytes = block_row; }
   else { *tally_mode = TM_GLOBAL; *lds_bytes = 0; }
+}
+
+int transport_choose(const Ctx& c, int* grid_blocks, int* tally_mode, size_t* lds_bytes) {
+  pick_tally(c, c.NB, tally_mode, lds_bytes);
   int per_cu = 0;
   hipError_t err;
 #define OCC(P, TM_, D)                                                                             \
   err = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_transport<P, TM_, D>, TPB, *lds_bytes)
 #define OCCD(P, TM_) do { if (c.is2d) OCC(P, TM_, true); else OCC(P, TM_, false); } while (0)
-#define OCCT(P) do { if (*tally_mode == TM_WAVEROW) OCCD(P, TM_WAVEROW); else OCCD(P, TM_GLOBAL); } while (0)
+#define OCCT(P) do { if (*tally_mode == TM_WAVEROW) OCCD(P, TM_WAVEROW);                          \
+                     else if (*tally_mode == TM_BLOCKROW) OCCD(P, TM_BLOCKROW);                   \
+                     else OCCD(P, TM_GLOBAL); } while (0)
   if (c.prec == 64) OCCT(64);
   else if (c.prec == 32) OCCT(32);
   else OCCT(16);
@@ -703,7 +739,9 @@ hipError_t launch_transport(Ctx& c, int grid, int tm, size_t lds) {
 #define TR(P, TM_, D)                                                                            \
   k_transport<P, TM_, D><<<grid, TPB, lds, c.stream>>>(s, g, c.d_sc, c.d_partial, c.d_dep)
 #define TRD(P, TM_) do { if (c.is2d) TR(P, TM_, true); else TR(P, TM_, false); } while (0)
-#define TRT(P) do { if (tm == TM_WAVEROW) TRD(P, TM_WAVEROW); else TRD(P, TM_GLOBAL); } while (0)
+#define TRT(P) do { if (tm == TM_WAVEROW) TRD(P, TM_WAVEROW);                                     \
+                    else if (tm == TM_BLOCKROW) TRD(P, TM_BLOCKROW);                              \
+                    else TRD(P, TM_GLOBAL); } while (0)
   if (c.prec == 64) TRT(64);
   else if (c.prec == 32) TRT(32);
   else TRT(16);
@@ -831,10 +869,7 @@ __global__ __launch_bounds__(TPB) void k_clean_scatter(SoA src, SoA dst, int64_t
 }
 
 int clean_choose(const Ctx& c, int* grid_blocks, int* tally_mode, size_t* lds_bytes) {
-  const size_t acc = c.prec == 64 ? 8 : 4;
-  const size_t wave_rows = (size_t)NWAVES * c.C * 2 * acc;
-  if (wave_rows <= 64 * 1024) { *tally_mode = TM_WAVEROW; *lds_bytes = wave_rows; }
-  else { *tally_mode = TM_GLOBAL; *lds_bytes = 0; }
+  pick_tally(c, c.C, tally_mode, lds_bytes);
   *grid_blocks = 2 * c.num_cu;
   return 0;
 }
@@ -861,7 +896,9 @@ hipError_t launch_clean(Ctx& c, int64_t n_total) {
   k_clean_scatter<P, TM_, D><<<grid, TPB, lds, c.stream>>>(src, dst, n_total, ntiles, c.d_tile, g, \
                                                           c.d_partial, c.d_rad)
 #define CLD(P, TM_) do { if (c.is2d) CL(P, TM_, true); else CL(P, TM_, false); } while (0)
-#define CLT(P) do { if (tm == TM_WAVEROW) CLD(P, TM_WAVEROW); else CLD(P, TM_GLOBAL); } while (0)
+#define CLT(P) do { if (tm == TM_WAVEROW) CLD(P, TM_WAVEROW);                                     \
+                    else if (tm == TM_BLOCKROW) CLD(P, TM_BLOCKROW);                              \
+                    else CLD(P, TM_GLOBAL); } while (0)
   if (c.prec == 64) CLT(64);
   else if (c.prec == 32) CLT(32);
   else CLT(16);
