@@ -207,7 +207,8 @@ def c4(seed=4):
     return f"""NAME = Thick Thin Blocks
 # Workload 4: 2000 cells over [0,10]; 40 alternating 50-cell blocks, thin sigma_a
 # log-uniform in [0.1,1], thick log-uniform in [1e2,1e4], sigma_s = sigma_a/2 (seed {seed}).
-# Left surface T = 1.  Pinned: bee 0.1, T_INIT 0.01, dt 1e-3 to t = 0.2, CELLMIN 1.
+# Left surface T = 1.  Pinned: bee 0.1, T_INIT 1.0 (hot, so the thick blocks are diffusive:
+# collision optical thickness up to ~75 per cell), dt 1e-3 to t = 0.05, CELLMIN 1.
 PRECISION = FLOAT32
 SEED = 4444
 GEOMETRY = 1D
@@ -228,11 +229,11 @@ LEFTBC = VACUUM
 RIGHTBC = VACUUM
 TIMESTEPPING = CONSTANT
 DT = 0.001
-ENDTIME = 0.2
+ENDTIME = 0.05
 NINPUT = 2000000
 NMAX = 20000000
 CELLMIN = 1
-T_INIT = 0.01
+T_INIT = 1.0
 T_SURFACE_VALS = [1.0,0.0]
 T_SURFACE_REGS = []
 PHYS_C = 299.8

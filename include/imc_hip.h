@@ -63,6 +63,7 @@ typedef struct {
   int64_t segments;               /* event-loop passes (the reference's `iterations`) */
   int32_t transport_tally_mode;   /* 0 wave rows, 1 block row (LDS atomics), 2 global atomics */
   int32_t transport_grid;         /* persistent transport blocks of the last launch */
+  int64_t surrogate_events;       /* thick-cell walks replaced by the surrogate */
 } imc_timing;
 
 /* Creates a context: MixedPrecisionIMC.jl:117 (particles = Vector{Vector{}}()) plus the
@@ -114,6 +115,34 @@ int imc_synchronize(imc_ctx* ctx);
 
 /* Bytes of particle state per particle held between kernels (S_state). */
 int imc_state_bytes(const imc_ctx* ctx, int32_t* bytes);
+
+/* ---- Learned thick-cell surrogate -------------------------------------------------
+ * Fills the slot of Transport.MC_RW (imc_transport.jl:212-479, deck RANDOMWALK = TRUE):
+ * a 6 -> 64 -> 64 -> 4 network (BF16 matrix-core inference and training, FP32
+ * accumulation and master weights) replaces the random walk of a particle inside a 1D
+ * cell whose collision optical thickness lies in the training box.  Training data are
+ * plain single-cell histories generated on the GPU.  Features and outputs are described
+ * in mixedprecisionimc.jl_amd/csrc/imc_mlp.h.  Parameter vector layout (4868 floats):
+ * W1[6][64], b1[64], W2[64][64], b2[64], W3[64][4], b3[4] (weights stored in x out). */
+#define IMC_SURROGATE_NPARAM 4868
+typedef struct {
+  float tau_cell_min, tau_cell_max;   /* collision optical thickness sigma_c*dx of the cell */
+  float tau_b_min, tau_b_max;         /* path budget in mean free paths */
+  float kappa_min, kappa_max;         /* f*sigma_a / sigma_c */
+  float depth_min;                    /* applied only this many mean free paths from both faces */
+} imc_surrogate_box;
+
+int imc_surrogate_configure(imc_ctx* ctx, const imc_surrogate_box* box);
+int imc_surrogate_init(imc_ctx* ctx, uint32_t seed);
+int imc_surrogate_generate(imc_ctx* ctx, int64_t n_samples, uint32_t seed, int64_t* events);
+int imc_surrogate_dataset(imc_ctx* ctx, float* features, float* targets, int64_t n);
+int imc_surrogate_train(imc_ctx* ctx, int32_t epochs, int32_t batch, float lr, float* loss_per_epoch);
+int imc_surrogate_set_params(imc_ctx* ctx, const float* params);
+int imc_surrogate_get_params(imc_ctx* ctx, float* params);
+int imc_surrogate_forward(imc_ctx* ctx, const float* features, int64_t n, float* outputs);
+int imc_surrogate_gradient(imc_ctx* ctx, const float* features, const float* targets, int64_t n,
+                           float* grad, float* loss);
+int imc_surrogate_enable(imc_ctx* ctx, int32_t on);
 
 #ifdef __cplusplus
 }

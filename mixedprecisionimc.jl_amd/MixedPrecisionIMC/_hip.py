@@ -66,7 +66,18 @@ class ImcTiming(ctypes.Structure):
         ("segments", ctypes.c_int64),
         ("transport_tally_mode", ctypes.c_int32),
         ("transport_grid", ctypes.c_int32),
+        ("surrogate_events", ctypes.c_int64),
     ]
+
+
+class SurrogateBox(ctypes.Structure):
+    _fields_ = [("tau_cell_min", ctypes.c_float), ("tau_cell_max", ctypes.c_float),
+                ("tau_b_min", ctypes.c_float), ("tau_b_max", ctypes.c_float),
+                ("kappa_min", ctypes.c_float), ("kappa_max", ctypes.c_float),
+                ("depth_min", ctypes.c_float)]
+
+
+SURROGATE_NPARAM = 4868
 
 
 _lib: Optional[ctypes.CDLL] = None
@@ -93,7 +104,32 @@ SYMBOLS = {
     "imc_reset_timing": (ctypes.c_int, [_VP]),
     "imc_synchronize": (ctypes.c_int, [_VP]),
     "imc_state_bytes": (ctypes.c_int, [_VP, ctypes.POINTER(ctypes.c_int32)]),
+    "imc_surrogate_configure": (ctypes.c_int, [_VP, ctypes.POINTER(SurrogateBox)]),
+    "imc_surrogate_init": (ctypes.c_int, [_VP, ctypes.c_uint32]),
+    "imc_surrogate_generate": (ctypes.c_int, [_VP, ctypes.c_int64, ctypes.c_uint32, _I64P]),
+    "imc_surrogate_dataset": (ctypes.c_int, [_VP, ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float),
+                                             ctypes.c_int64]),
+    "imc_surrogate_train": (ctypes.c_int, [_VP, ctypes.c_int32, ctypes.c_int32, ctypes.c_float,
+                                           ctypes.POINTER(ctypes.c_float)]),
+    "imc_surrogate_set_params": (ctypes.c_int, [_VP, ctypes.POINTER(ctypes.c_float)]),
+    "imc_surrogate_get_params": (ctypes.c_int, [_VP, ctypes.POINTER(ctypes.c_float)]),
+    "imc_surrogate_forward": (ctypes.c_int, [_VP, ctypes.POINTER(ctypes.c_float), ctypes.c_int64,
+                                             ctypes.POINTER(ctypes.c_float)]),
+    "imc_surrogate_gradient": (ctypes.c_int, [_VP, ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float),
+                                              ctypes.c_int64, ctypes.POINTER(ctypes.c_float),
+                                              ctypes.POINTER(ctypes.c_float)]),
+    "imc_surrogate_enable": (ctypes.c_int, [_VP, ctypes.c_int32]),
 }
+
+_FP = ctypes.POINTER(ctypes.c_float)
+
+
+def _fp(a):
+    return a.ctypes.data_as(_FP)
+
+
+def _f32(a) -> np.ndarray:
+    return np.ascontiguousarray(np.asarray(a, dtype=np.float32))
 
 
 def lib() -> ctypes.CDLL:
@@ -255,3 +291,54 @@ class Engine:
         b = ctypes.c_int32()
         self._check(lib().imc_state_bytes(self._h, ctypes.byref(b)))
         return b.value
+
+    # -- learned thick-cell surrogate (slot of Transport.MC_RW) -----------------
+    def surrogate_configure(self, tau_cell=(20.0, 200.0), tau_b=(2.0, 1e4), kappa=(1e-5, 4.0), depth_min=3.0):
+        box = SurrogateBox(tau_cell[0], tau_cell[1], tau_b[0], tau_b[1], kappa[0], kappa[1], depth_min)
+        self._check(lib().imc_surrogate_configure(self._h, ctypes.byref(box)))
+
+    def surrogate_init(self, seed: int = 0):
+        self._check(lib().imc_surrogate_init(self._h, int(seed) & 0xFFFFFFFF))
+
+    def surrogate_generate(self, n: int, seed: int = 0) -> int:
+        ev = ctypes.c_int64()
+        self._check(lib().imc_surrogate_generate(self._h, int(n), int(seed) & 0xFFFFFFFF, ctypes.byref(ev)))
+        return ev.value
+
+    def surrogate_dataset(self, n: int):
+        f = np.zeros((n, 6), np.float32)
+        t = np.zeros((n, 4), np.float32)
+        self._check(lib().imc_surrogate_dataset(self._h, _fp(f), _fp(t), int(n)))
+        return f, t
+
+    def surrogate_train(self, epochs: int, batch: int = 8192, lr: float = 3e-3) -> np.ndarray:
+        losses = np.zeros(epochs, np.float32)
+        self._check(lib().imc_surrogate_train(self._h, int(epochs), int(batch), float(lr), _fp(losses)))
+        return losses
+
+    def surrogate_params(self) -> np.ndarray:
+        p = np.zeros(SURROGATE_NPARAM, np.float32)
+        self._check(lib().imc_surrogate_get_params(self._h, _fp(p)))
+        return p
+
+    def surrogate_set_params(self, params):
+        p = _f32(params)
+        assert p.size == SURROGATE_NPARAM
+        self._check(lib().imc_surrogate_set_params(self._h, _fp(p)))
+
+    def surrogate_forward(self, features) -> np.ndarray:
+        f = _f32(features).reshape(-1, 6)
+        out = np.zeros((f.shape[0], 4), np.float32)
+        self._check(lib().imc_surrogate_forward(self._h, _fp(f), f.shape[0], _fp(out)))
+        return out
+
+    def surrogate_gradient(self, features, targets):
+        f = _f32(features).reshape(-1, 6)
+        t = _f32(targets).reshape(-1, 4)
+        g = np.zeros(SURROGATE_NPARAM, np.float32)
+        loss = ctypes.c_float()
+        self._check(lib().imc_surrogate_gradient(self._h, _fp(f), _fp(t), f.shape[0], _fp(g), ctypes.byref(loss)))
+        return g, loss.value
+
+    def surrogate_enable(self, on: bool = True):
+        self._check(lib().imc_surrogate_enable(self._h, int(bool(on))))

@@ -9,10 +9,12 @@
 #include <cstdio>
 #include <cstring>
 #include <new>
+#include <random>
 #include <vector>
 
 #include "imc_device.h"
 #include "imc_internal.h"
+#include "imc_mlp.h"
 
 using namespace imc;
 
@@ -224,7 +226,8 @@ int imc_destroy(imc_ctx* h) {
   void* ptrs[] = {c.d_dx, c.d_dy, c.d_dxs, c.d_dys, c.d_dxw, c.d_dyw, c.d_temp, c.d_fleck, c.d_sa, c.d_ss,
                   c.d_radsrc, c.d_tsurf, c.d_sigcol, c.d_sigdep, c.d_ent_e, c.d_ent_esc, c.d_ent_term,
                   c.d_ent_k, c.d_ent_cnt, c.d_ent_off, c.d_red, c.d_emitted, c.d_dep, c.d_rad, c.d_sc,
-                  c.d_partial, c.d_tile};
+                  c.d_partial, c.d_tile, c.d_surr_feat, c.d_surr_targ, c.d_surr_params, c.d_surr_grad,
+                  c.d_surr_m, c.d_surr_v, c.d_surr_wbf, c.d_surr_bias, c.d_surr_loss, c.d_surr_events};
   for (void* p : ptrs) (void)hipFree(p);
   hipEvent_t evs[] = {c.ev0, c.ev1, c.evk0, c.evk1};
   for (hipEvent_t e : evs)
@@ -351,6 +354,7 @@ int imc_transport(imc_ctx* h, double* energydep, double* lostenergy, int64_t* se
   c.timing.histories += c.h_sc.n_total;
   c.timing.census_out += (int64_t)c.h_sc.census_out;
   c.timing.segments += (int64_t)c.h_sc.segments;
+  c.timing.surrogate_events += (int64_t)c.h_sc.surr_events;
   c.timing.transport_tally_mode = tm;
   c.timing.transport_grid = grid;
   if (lostenergy) *lostenergy = lost;
@@ -514,6 +518,217 @@ int imc_synchronize(imc_ctx* h) {
 int imc_state_bytes(const imc_ctx* h, int32_t* bytes) {
   if (!h || !bytes) return IMC_EINVAL;
   *bytes = state_bytes(h->c.prec, h->c.is2d);
+  return IMC_OK;
+}
+
+// ---------------------------------------------------------------------------------
+// learned thick-cell surrogate
+// ---------------------------------------------------------------------------------
+static int surr_alloc(Ctx& c) {
+  if (c.d_surr_params) return IMC_OK;
+  bool ok = true;
+  ok &= dalloc(&c.d_surr_params, 4 * (size_t)MLP_NPARAM) == hipSuccess;
+  ok &= dalloc(&c.d_surr_grad, 4 * (size_t)MLP_NPARAM) == hipSuccess;
+  ok &= dalloc(&c.d_surr_m, 4 * (size_t)MLP_NPARAM) == hipSuccess;
+  ok &= dalloc(&c.d_surr_v, 4 * (size_t)MLP_NPARAM) == hipSuccess;
+  ok &= dalloc(&c.d_surr_wbf, 2 * (size_t)BF_TOTAL) == hipSuccess;
+  ok &= dalloc(&c.d_surr_bias, 4 * (size_t)BIAS_TOTAL) == hipSuccess;
+  ok &= dalloc(&c.d_surr_loss, 8) == hipSuccess;
+  ok &= dalloc(&c.d_surr_events, 8) == hipSuccess;
+  if (!ok) return fail(c, IMC_ENOMEM, "surrogate allocation failed");
+  HIPCHK(launch_zero(c, c.d_surr_params, 4 * (size_t)MLP_NPARAM), "zero params");
+  HIPCHK(launch_zero(c, c.d_surr_grad, 4 * (size_t)MLP_NPARAM), "zero grad");
+  HIPCHK(launch_zero(c, c.d_surr_m, 4 * (size_t)MLP_NPARAM), "zero m");
+  HIPCHK(launch_zero(c, c.d_surr_v, 4 * (size_t)MLP_NPARAM), "zero v");
+  HIPCHK(surr_make_weights(c), "surrogate weights");
+  HIPCHK(hipStreamSynchronize(c.stream), "surrogate alloc sync");
+  return IMC_OK;
+}
+
+int imc_surrogate_configure(imc_ctx* h, const imc_surrogate_box* box) {
+  if (!h || !box) return IMC_EINVAL;
+  Ctx& c = h->c;
+  const imc_surrogate_box& b = *box;
+  if (!(b.tau_cell_min > 0 && b.tau_cell_max > b.tau_cell_min && b.tau_b_min > 0 && b.tau_b_max > b.tau_b_min &&
+        b.kappa_min > 0 && b.kappa_max > b.kappa_min && b.depth_min >= 0 && 2 * b.depth_min < b.tau_cell_min))
+    return fail(c, IMC_EINVAL, "surrogate box bounds must be positive and increasing (2 depth_min < tau_cell_min)");
+  c.surr_box = SurrBox{b.tau_cell_min, b.tau_cell_max, b.tau_b_min, b.tau_b_max, b.kappa_min, b.kappa_max, b.depth_min};
+  return surr_alloc(c);
+}
+
+int imc_surrogate_set_params(imc_ctx* h, const float* params) {
+  if (!h || !params) return IMC_EINVAL;
+  Ctx& c = h->c;
+  int rc = surr_alloc(c);
+  if (rc) return rc;
+  HIPCHK(hipMemcpyAsync(c.d_surr_params, params, 4 * (size_t)MLP_NPARAM, hipMemcpyHostToDevice, c.stream), "params H2D");
+  HIPCHK(surr_make_weights(c), "surrogate weights");
+  HIPCHK(hipStreamSynchronize(c.stream), "params sync");
+  return IMC_OK;
+}
+
+int imc_surrogate_get_params(imc_ctx* h, float* params) {
+  if (!h || !params) return IMC_EINVAL;
+  Ctx& c = h->c;
+  int rc = surr_alloc(c);
+  if (rc) return rc;
+  HIPCHK(hipMemcpyAsync(params, c.d_surr_params, 4 * (size_t)MLP_NPARAM, hipMemcpyDeviceToHost, c.stream), "params D2H");
+  HIPCHK(hipStreamSynchronize(c.stream), "params sync");
+  return IMC_OK;
+}
+
+// He-normal initialisation of the weights, zero biases; resets the Adam state.
+int imc_surrogate_init(imc_ctx* h, uint32_t seed) {
+  if (!h) return IMC_EINVAL;
+  Ctx& c = h->c;
+  int rc = surr_alloc(c);
+  if (rc) return rc;
+  std::vector<float> p(MLP_NPARAM, 0.0f);
+  std::mt19937 gen(seed);
+  std::normal_distribution<float> nd(0.0f, 1.0f);
+  const float s1 = std::sqrt(2.0f / MLP_IN), s2 = std::sqrt(2.0f / MLP_H), s3 = std::sqrt(1.0f / MLP_H);
+  for (int i = 0; i < MLP_IN * MLP_H; ++i) p[OFF_W1 + i] = s1 * nd(gen);
+  for (int i = 0; i < MLP_H * MLP_H; ++i) p[OFF_W2 + i] = s2 * nd(gen);
+  for (int i = 0; i < MLP_H * MLP_OUT; ++i) p[OFF_W3 + i] = s3 * nd(gen);
+  HIPCHK(launch_zero(c, c.d_surr_m, 4 * (size_t)MLP_NPARAM), "zero m");
+  HIPCHK(launch_zero(c, c.d_surr_v, 4 * (size_t)MLP_NPARAM), "zero v");
+  HIPCHK(launch_zero(c, c.d_surr_grad, 4 * (size_t)MLP_NPARAM), "zero grad");
+  c.surr_step = 0;
+  return imc_surrogate_set_params(h, p.data());
+}
+
+int imc_surrogate_generate(imc_ctx* h, int64_t n, uint32_t seed, int64_t* events) {
+  if (!h) return IMC_EINVAL;
+  Ctx& c = h->c;
+  if (n <= 0) return fail(c, IMC_EINVAL, "n_samples must be positive");
+  int rc = surr_alloc(c);
+  if (rc) return rc;
+  if (n > c.surr_cap) {
+    (void)hipFree(c.d_surr_feat);
+    (void)hipFree(c.d_surr_targ);
+    c.d_surr_feat = nullptr; c.d_surr_targ = nullptr; c.surr_cap = 0;
+    if (dalloc(&c.d_surr_feat, 4 * (size_t)n * MLP_IN) != hipSuccess ||
+        dalloc(&c.d_surr_targ, 4 * (size_t)n * 4) != hipSuccess)
+      return fail(c, IMC_ENOMEM, "surrogate dataset allocation failed");
+    c.surr_cap = n;
+  }
+  HIPCHK(launch_zero(c, c.d_surr_events, 8), "zero events");
+  HIPCHK(surr_generate(c, n, seed, c.d_surr_events), "generate kernel");
+  unsigned long long ev = 0;
+  HIPCHK(hipMemcpyAsync(&ev, c.d_surr_events, 8, hipMemcpyDeviceToHost, c.stream), "events D2H");
+  HIPCHK(hipStreamSynchronize(c.stream), "generate sync");
+  c.surr_n = n;
+  if (events) *events = (int64_t)ev;
+  return IMC_OK;
+}
+
+int imc_surrogate_dataset(imc_ctx* h, float* features, float* targets, int64_t n) {
+  if (!h) return IMC_EINVAL;
+  Ctx& c = h->c;
+  if (n > c.surr_n) return fail(c, IMC_EINVAL, "more samples requested than generated");
+  if (features)
+    HIPCHK(hipMemcpyAsync(features, c.d_surr_feat, 4 * (size_t)n * MLP_IN, hipMemcpyDeviceToHost, c.stream), "D2H");
+  if (targets)
+    HIPCHK(hipMemcpyAsync(targets, c.d_surr_targ, 4 * (size_t)n * 4, hipMemcpyDeviceToHost, c.stream), "D2H");
+  HIPCHK(hipStreamSynchronize(c.stream), "dataset sync");
+  return IMC_OK;
+}
+
+// Mini-batch Adam over the generated set, cosine learning-rate decay to lr/20.
+int imc_surrogate_train(imc_ctx* h, int32_t epochs, int32_t batch, float lr, float* loss_per_epoch) {
+  if (!h) return IMC_EINVAL;
+  Ctx& c = h->c;
+  if (c.surr_n <= 0) return fail(c, IMC_EINVAL, "imc_surrogate_generate must be called first");
+  if (epochs <= 0 || batch < 64 || (batch % 64) != 0 || !(lr > 0))
+    return fail(c, IMC_EINVAL, "epochs > 0, batch a positive multiple of 64, lr > 0 required");
+  const int64_t steps_per_epoch = c.surr_n / batch;
+  if (steps_per_epoch <= 0) return fail(c, IMC_EINVAL, "batch larger than the data set");
+  const int64_t total = steps_per_epoch * epochs;
+  int64_t it = 0;
+  for (int e = 0; e < epochs; ++e) {
+    HIPCHK(launch_zero(c, c.d_surr_loss, 8), "zero loss");
+    for (int64_t b = 0; b < steps_per_epoch; ++b, ++it) {
+      HIPCHK(surr_grad(c, c.d_surr_feat, c.d_surr_targ, b * batch, batch, 1.0f / batch, c.d_surr_grad,
+                       c.d_surr_loss), "gradient kernel");
+      const float frac = (float)it / (float)std::max<int64_t>(total - 1, 1);
+      const float lr_t = lr * (0.05f + 0.95f * 0.5f * (1.0f + std::cos(3.14159265f * frac)));
+      HIPCHK(surr_adam(c, lr_t, ++c.surr_step), "adam kernel");
+      HIPCHK(surr_make_weights(c), "surrogate weights");
+    }
+    float lsum = 0.f;
+    HIPCHK(hipMemcpyAsync(&lsum, c.d_surr_loss, 4, hipMemcpyDeviceToHost, c.stream), "loss D2H");
+    HIPCHK(hipStreamSynchronize(c.stream), "train sync");
+    if (loss_per_epoch) loss_per_epoch[e] = lsum / (float)(steps_per_epoch * batch);
+  }
+  return IMC_OK;
+}
+
+int imc_surrogate_forward(imc_ctx* h, const float* features, int64_t n, float* outputs) {
+  if (!h || !features || !outputs || n <= 0) return IMC_EINVAL;
+  Ctx& c = h->c;
+  int rc = surr_alloc(c);
+  if (rc) return rc;
+  float *df = nullptr, *dout = nullptr;
+  if (dalloc(&df, 4 * (size_t)n * MLP_IN) != hipSuccess || dalloc(&dout, 4 * (size_t)n * MLP_OUT) != hipSuccess) {
+    (void)hipFree(df);
+    return fail(c, IMC_ENOMEM, "forward buffers");
+  }
+  hipError_t e = hipMemcpyAsync(df, features, 4 * (size_t)n * MLP_IN, hipMemcpyHostToDevice, c.stream);
+  if (e == hipSuccess) e = surr_forward(c, df, n, dout);
+  if (e == hipSuccess) e = hipMemcpyAsync(outputs, dout, 4 * (size_t)n * MLP_OUT, hipMemcpyDeviceToHost, c.stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c.stream);
+  (void)hipFree(df);
+  (void)hipFree(dout);
+  if (e != hipSuccess) return hipfail(c, e, "surrogate forward");
+  return IMC_OK;
+}
+
+// Gradient of the mean training loss over n samples (no parameter update).
+int imc_surrogate_gradient(imc_ctx* h, const float* features, const float* targets, int64_t n, float* grad,
+                           float* loss) {
+  if (!h || !features || !targets || !grad || n <= 0) return IMC_EINVAL;
+  Ctx& c = h->c;
+  int rc = surr_alloc(c);
+  if (rc) return rc;
+  float *df = nullptr, *dt = nullptr, *dg = nullptr, *dl = nullptr;
+  if (dalloc(&df, 4 * (size_t)n * MLP_IN) != hipSuccess || dalloc(&dt, 4 * (size_t)n * 4) != hipSuccess ||
+      dalloc(&dg, 4 * (size_t)MLP_NPARAM) != hipSuccess || dalloc(&dl, 8) != hipSuccess) {
+    (void)hipFree(df); (void)hipFree(dt); (void)hipFree(dg);
+    return fail(c, IMC_ENOMEM, "gradient buffers");
+  }
+  float lsum = 0.f;
+  hipError_t e = hipMemcpyAsync(df, features, 4 * (size_t)n * MLP_IN, hipMemcpyHostToDevice, c.stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(dt, targets, 4 * (size_t)n * 4, hipMemcpyHostToDevice, c.stream);
+  if (e == hipSuccess) e = launch_zero(c, dg, 4 * (size_t)MLP_NPARAM);
+  if (e == hipSuccess) e = launch_zero(c, dl, 8);
+  if (e == hipSuccess) e = surr_grad(c, df, dt, 0, n, 1.0f / (float)n, dg, dl);
+  if (e == hipSuccess) e = hipMemcpyAsync(grad, dg, 4 * (size_t)MLP_NPARAM, hipMemcpyDeviceToHost, c.stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(&lsum, dl, 4, hipMemcpyDeviceToHost, c.stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c.stream);
+  (void)hipFree(df); (void)hipFree(dt); (void)hipFree(dg); (void)hipFree(dl);
+  if (e != hipSuccess) return hipfail(c, e, "surrogate gradient");
+  if (loss) *loss = lsum / (float)n;
+  return IMC_OK;
+}
+
+int imc_surrogate_enable(imc_ctx* h, int32_t on) {
+  if (!h) return IMC_EINVAL;
+  Ctx& c = h->c;
+  if (on && c.is2d) return fail(c, IMC_EINVAL, "the surrogate is available for 1D geometry");
+  int rc = surr_alloc(c);
+  if (rc) return rc;
+  c.surr_enabled = on != 0;
+  // partial buffer must cover the transport grid of the active configuration
+  int grid, tm; size_t lds;
+  transport_choose(c, &grid, &tm, &lds);
+  const size_t acc = c.prec == 64 ? 8 : 4;
+  const size_t need = (size_t)grid * c.NB * 2 * acc;
+  if (need > c.partial_bytes) {
+    (void)hipFree(c.d_partial);
+    c.d_partial = nullptr;
+    if (dalloc(&c.d_partial, need) != hipSuccess) return fail(c, IMC_ENOMEM, "tally partial allocation failed");
+    c.partial_bytes = need;
+  }
   return IMC_OK;
 }
 

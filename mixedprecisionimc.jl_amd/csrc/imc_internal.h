@@ -30,8 +30,16 @@ struct Scalars {
   double E_tot;                // mesh.totalenergy
   unsigned long long segments; // event-loop passes of the last transport
   unsigned long long census_out;
+  unsigned long long surr_events;  // surrogate evaluations of the last transport
   uint32_t work;               // transport work-queue cursor
   uint32_t pad;
+};
+
+struct SurrBox {               // training box of the thick-cell surrogate (see imc_mlp.h)
+  float tau_cell_min, tau_cell_max;
+  float tau_b_min, tau_b_max;
+  float kappa_min, kappa_max;
+  float depth_min;             // distance to the nearest face, in mean free paths
 };
 
 struct Geo {                   // kernel view of the geometry and step coefficients
@@ -92,6 +100,22 @@ struct Ctx {
   int64_t* d_tile = nullptr;       // clean: per-tile counts/offsets
   int64_t tile_len = 0;
 
+  // learned thick-cell surrogate
+  SurrBox surr_box{20.f, 200.f, 2.f, 1e4f, 1e-5f, 4.f, 3.f};
+  bool surr_enabled = false;
+  int64_t surr_cap = 0, surr_n = 0;
+  float* d_surr_feat = nullptr;      // [surr_cap][6]
+  float* d_surr_targ = nullptr;      // [surr_cap][4]
+  float* d_surr_params = nullptr;    // 4868 FP32 master parameters
+  float* d_surr_grad = nullptr;
+  float* d_surr_m = nullptr;
+  float* d_surr_v = nullptr;
+  uint16_t* d_surr_wbf = nullptr;    // BF16 inference copy
+  float* d_surr_bias = nullptr;      // FP32 biases
+  float* d_surr_loss = nullptr;
+  unsigned long long* d_surr_events = nullptr;
+  int surr_step = 0;
+
   // timing
   hipEvent_t ev0 = nullptr, ev1 = nullptr, evk0 = nullptr, evk1 = nullptr;
   imc_timing timing{};
@@ -112,6 +136,12 @@ hipError_t launch_reduce_partials(Ctx& c, int nblocks, int tally_mode, int nbins
 hipError_t launch_zero(Ctx& c, void* p, size_t bytes);
 int transport_choose(const Ctx& c, int* grid_blocks, int* tally_mode, size_t* lds_bytes);
 int clean_choose(const Ctx& c, int* grid_blocks, int* tally_mode, size_t* lds_bytes);
+hipError_t surr_generate(Ctx& c, int64_t n, uint32_t seed, unsigned long long* d_events);
+hipError_t surr_make_weights(Ctx& c);
+hipError_t surr_grad(Ctx& c, const float* feat, const float* targ, int64_t n0, int64_t n, float inv_batch,
+                     float* grad, float* loss);
+hipError_t surr_adam(Ctx& c, float lr, int step);
+hipError_t surr_forward(Ctx& c, const float* feat, int64_t n, float* out);
 
 }  // namespace imc
 

@@ -14,6 +14,7 @@
 
 #include "imc_device.h"
 #include "imc_internal.h"
+#include "imc_mlp.h"
 
 namespace imc {
 
@@ -501,16 +502,49 @@ hipError_t launch_reduce_partials(Ctx& c, int nblocks, int tally_mode, int nbins
 // Persistent blocks; each wave pulls CHUNK-slot ranges from a device work counter and
 // refills a lane as soon as its history ends.  History in slot p uses Philox counters
 // (p, event, TAG_EVENT, redraw) keyed by (seed, step).
+//
+// SURR (1D): a lane whose cell has collision optical thickness sigma_c*dx inside the
+// surrogate's training box takes one network evaluation instead of its in-cell random
+// walk (the slot of Transport.MC_RW, imc_transport.jl:286-372): with the predicted
+// P(stay) it stays until its path budget ends (census with weight E exp(-f sigma_a c(dt-t)),
+// or death by the 1% cut-off), otherwise it leaves through the right face with
+// P(right | exit), after the predicted mean path, with the predicted expected deposit
+// fraction, in an outward direction drawn from mu (1 + 1.5 mu).  Only particles at least
+// depth_min mean free paths from both faces use it; plain events resolve the layer next
+// to the faces.  Deposits are E_in - E_out, so every
+// history conserves energy exactly.
 // ---------------------------------------------------------------------------------
-template <int P, int TM, bool TWO_D>
+struct SurrArgs {
+  const uint16_t* wbf;
+  const float* bias;
+  SurrBox box;
+  int tally_bytes;
+};
+
+constexpr uint32_t TAG_SURR = 3u;
+constexpr size_t SURR_LDS = (size_t)BF_TOTAL * 2 + 544 + (size_t)NWAVES * sizeof(MlpScratch);
+
+template <int P, int TM, bool TWO_D, bool SURR>
 __global__ __launch_bounds__(TPB) void k_transport(SoA s, Geo g, Scalars* sc, void* partial_v,
-                                                   double* gbins) {
+                                                   double* gbins, SurrArgs sa) {
   using T = typename Prec<P>::T;
   using W = typename Prec<P>::W;
   using Acc = typename Prec<P>::Acc;
+  static_assert(!(SURR && TWO_D), "the surrogate path is 1D");
   extern __shared__ __align__(16) unsigned char smem[];
   Acc* lds = (Acc*)smem;
   const int NB = g.NB, C = g.C, nx = g.nx, ny = g.ny;
+  uint16_t* s_w = nullptr;
+  float* s_b = nullptr;
+  MlpScratch* s_scr = nullptr;
+  if (SURR) {
+    unsigned char* base = smem + sa.tally_bytes;
+    s_w = (uint16_t*)base;
+    s_b = (float*)(base + (size_t)BF_TOTAL * 2);
+    s_scr = (MlpScratch*)(base + (size_t)BF_TOTAL * 2 + 544);
+    for (int i = threadIdx.x; i < BF_TOTAL; i += blockDim.x) s_w[i] = sa.wbf[i];
+    for (int i = threadIdx.x; i < BIAS_TOTAL; i += blockDim.x) s_b[i] = sa.bias[i];
+  }
   tally_init<Acc, TM>(lds, NB);
   Tally<Acc, TM> tal{lds, gbins, NB};
 
@@ -541,7 +575,7 @@ __global__ __launch_bounds__(TPB) void k_transport(SoA s, Geo g, Scalars* sc, vo
   W E = (W)0, Emin = (W)0;
   uint32_t cur = 0, cend = 0;
   bool done = false;
-  unsigned long long segs = 0, cens = 0;
+  unsigned long long segs = 0, cens = 0, nsurr = 0;
 
   for (;;) {
     // ---- lane refill from the wave's chunk ----
@@ -577,7 +611,138 @@ __global__ __launch_bounds__(TPB) void k_transport(SoA s, Geo g, Scalars* sc, vo
 
     int bin = -1, lbin = -1;
     Acc val = (Acc)0, lval = (Acc)0;
-    if (has) {
+
+    // ---- surrogate decision (1D, thick cells inside the training box) ----
+    bool want = false;
+    float feat[MLP_IN] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    float sout[MLP_OUT] = {0.f, 0.f, 0.f, 0.f};
+    float s_ST = 0.f, s_Scut = 0.f, s_SB = 0.f, s_sc = 0.f;
+    if (SURR) {
+      if (has) {
+        const float scf = (float)sigcol[ix], Lf = (float)dxs[ix];
+        const float tau_cell = scf * Lf;
+        if (tau_cell >= sa.box.tau_cell_min && tau_cell <= sa.box.tau_cell_max) {
+          const float sdf = (float)sigdep[ix];
+          const float ST = (float)(cT * (dtT - t) * sT);
+          const float Scut = sdf > 0.f ? __logf((float)E / (float)Emin) / sdf : INFINITY;
+          const float SB = fminf(ST, Scut);
+          const float tau_b = scf * SB, kappa = sdf / scf;
+          const float depth = scf * fminf((float)x, Lf - (float)x);
+          if (depth >= sa.box.depth_min && surr_in_box(sa.box, tau_cell, tau_b, kappa)) {
+            want = true;
+            s_ST = ST; s_Scut = Scut; s_SB = SB; s_sc = scf;
+            surr_features(sa.box, tau_cell, (float)x / Lf, (float)mu, tau_b, kappa, feat);
+          }
+        }
+      }
+      // Batch the network: evaluate once at least 16 lanes want it, or when no lane of the
+      // wave has plain work left; until then wanting lanes wait (they do not advance).
+      const int nwant = __popcll(__ballot(want));
+      if (nwant > 0 && (nwant >= 16 || nwant == __popcll(__ballot(has)))) {
+        mlp_forward_wave(s_w, s_b, s_scr + (threadIdx.x >> 6), want, feat, sout);
+      } else if (want) {
+        want = false;
+        has = false;          // sits this pass out (restored below)
+        s_ST = -1.f;
+      }
+    }
+
+    if (SURR && want) {
+      ++segs;
+      ++nsurr;
+      const u4 q = philox(slot, ev, TAG_SURR, 0u, g.seed, g.step);
+      bin = k * C + ix;
+      if (u24(q.r[0]) < sout[0]) {                           // stays until the budget ends
+        if (s_Scut < s_ST) {                                  // 1% cut-off first: all deposited
+          val = (Acc)(E / dxw[ix]);
+          CK[slot] = DEAD;
+          has = false;
+        } else {                                              // census inside the cell
+          const W Enew = E * Tm<W>::exp_(-(sigdep[ix] * (W)(cT * (dtT - t) * sT)));
+          val = (Acc)((E - Enew) / dxw[ix]);
+          E = Enew;
+          // position of a walker that has not left the cell: diffusion from x0 with absorbing
+          // faces, spread sigma^2 = 2 S / (3 sigma_c).  Narrow spread: rejection against the
+          // first-order image terms, p(x) ~ G(x-x0) - G(x+x0) - G(x-2L+x0); wide spread: the
+          // fundamental mode sin(pi x / L).
+          const float Lf = (float)dxs[ix], x0 = (float)x;
+          const float sig = sqrtf(2.0f * s_ST / (3.0f * s_sc));
+          float xn;
+          if (sig < 0.5f * Lf) {
+            xn = x0;
+            for (uint32_t at = 1; at <= 16; ++at) {
+              const u4 r = philox(slot, ev, TAG_SURR, at, g.seed, g.step);
+              const float gz = sqrtf(-2.0f * __logf((float)((r.r[0] >> 8) + 1) * 0x1p-24f)) *
+                               __cosf(6.2831853f * u24(r.r[1]));
+              const float xc = x0 + sig * gz;
+              if (xc < 0.f || xc > Lf) continue;
+              const float inv = 1.0f / (2.0f * sig * sig);
+              const float e0 = (xc - x0) * (xc - x0) * inv;
+              const float im = __expf(e0 - (xc + x0) * (xc + x0) * inv) +
+                               __expf(e0 - (xc - 2.0f * Lf + x0) * (xc - 2.0f * Lf + x0) * inv);
+              xn = xc;
+              if (u24(r.r[2]) < 1.0f - im) break;
+            }
+          } else {
+            xn = Lf * 0.318309886f * acosf(1.0f - 2.0f * u24(q.r[2]));
+          }
+          xn = fminf(fmaxf(xn, 0.0f), Lf);
+          float m = 1.0f - 2.0f * u24(q.r[1]);
+          if (m == 0.0f) m = 0x1p-24f;
+          CK[slot] = (uint32_t)ix | ((uint32_t)k << KSHIFT);
+          X[slot] = Tm<T>::from((double)xn);
+          MU[slot] = Tm<T>::from((double)m);
+          TT[slot] = zero;
+          EE[slot] = E;
+          ++cens;
+          has = false;
+        }
+      } else {                                                // leaves through a face
+        const bool right = u24(q.r[1]) < sout[1];
+        const T path = Tm<T>::from((double)(sout[2] * s_SB));
+        const W Enew = E * (W)(1.0f - sout[3]);
+        val = (Acc)((E - Enew) / dxw[ix]);
+        E = Enew;
+        t = t + (path / sT) / cT;
+        // outgoing direction cosine from a diffusive medium, p(mu) ~ mu (1 + 1.5 mu):
+        // Lambertian proposal mu = sqrt(u) accepted with probability (1 + 1.5 mu)/2.5
+        float m = sqrtf(fmaxf(u24(q.r[2]), 0x1p-24f));
+        for (uint32_t at = 17; at <= 32 && u24(q.r[3]) * 2.5f > 1.0f + 1.5f * m; ++at) {
+          const u4 r = philox(slot, ev, TAG_SURR, at, g.seed, g.step);
+          m = sqrtf(fmaxf(u24(r.r[0]), 0x1p-24f));
+          if (u24(r.r[1]) * 2.5f <= 1.0f + 1.5f * m) break;
+        }
+        bool dead = false;
+        if (E <= Emin) {                                      // below the cut-off: deposit here
+          val += (Acc)(E / dxw[ix]);
+          CK[slot] = DEAD;
+          has = false;
+        } else {
+          mu = Tm<T>::from(right ? (double)m : -(double)m);
+          x = right ? dxs[ix] : zero;
+          if (mu > zero) {                                    // face crossing as :130-170
+            if (ix == C - 1) {
+              if (g.bc[1] == IMC_REFLECT) mu = -mu; else dead = true;
+            }
+            if (!dead) { ix += 1; x = zero; }
+          }
+          if (!dead && mu < zero) {
+            if (ix == 0) {
+              if (g.bc[0] == IMC_REFLECT) mu = -mu; else dead = true;
+            } else {
+              ix -= 1; x = dxs[ix];
+            }
+          }
+          if (dead) {
+            lbin = NB - 1;
+            lval = (Acc)(E / (W)g.scales[k]);
+            CK[slot] = DEAD;
+            has = false;
+          }
+        }
+      }
+      ++ev;
+    } else if (has) {
       ++segs;
       const u4 q = philox(slot, ev, TAG_EVENT, 0u, g.seed, g.step);
       const int cell = TWO_D ? ix + nx * iy : ix;
@@ -629,28 +794,29 @@ __global__ __launch_bounds__(TPB) void k_transport(SoA s, Geo g, Scalars* sc, vo
           y = y + d * oy;
           if (d == dbx || d == dby) {                                         // :621-704
             face = true;
+            bool turned = false;
             if (dbx < dby) {
               if (ox > zero) {
                 if (ix == nx - 1) {
-                  if (g.bc[1] == IMC_REFLECT) mu = Tm<T>::atan2_(oy, -ox); else dead = true;
+                  if (g.bc[1] == IMC_REFLECT) { mu = Tm<T>::atan2_(oy, -ox); turned = true; } else dead = true;
                 } else { ix += 1; x = zero; }
               } else {
                 if (ix == 0) {
-                  if (g.bc[0] == IMC_REFLECT) mu = Tm<T>::atan2_(oy, -ox); else dead = true;
+                  if (g.bc[0] == IMC_REFLECT) { mu = Tm<T>::atan2_(oy, -ox); turned = true; } else dead = true;
                 } else { ix -= 1; x = dxs[ix]; }
               }
             } else {
               if (oy > zero) {
                 if (iy == ny - 1) {
-                  if (g.bc[2] == IMC_REFLECT) mu = Tm<T>::atan2_(-oy, ox); else dead = true;
+                  if (g.bc[2] == IMC_REFLECT) { mu = Tm<T>::atan2_(-oy, ox); turned = true; } else dead = true;
                 } else { iy += 1; y = zero; }
               } else {
                 if (iy == 0) {
-                  if (g.bc[3] == IMC_REFLECT) mu = Tm<T>::atan2_(-oy, ox); else dead = true;
+                  if (g.bc[3] == IMC_REFLECT) { mu = Tm<T>::atan2_(-oy, ox); turned = true; } else dead = true;
                 } else { iy -= 1; y = dys[iy]; }
               }
             }
-            if (!dead) { ox = Tm<T>::cos_(mu); oy = Tm<T>::sin_(mu); }
+            if (turned) { ox = Tm<T>::cos_(mu); oy = Tm<T>::sin_(mu); }
           }
         }
         if (dead) {
@@ -685,15 +851,18 @@ __global__ __launch_bounds__(TPB) void k_transport(SoA s, Geo g, Scalars* sc, vo
       }
       ++ev;
     }
+    if (SURR && s_ST < 0.f) has = true;     // a parked lane keeps its particle
     tal.add(bin, val);
     tal.add(lbin, lval);
   }
 
   segs = wave_sum(segs);
   cens = wave_sum(cens);
+  if (SURR) nsurr = wave_sum(nsurr);
   if (lane == 0) {
     atomicAdd(&sc->segments, segs);
     atomicAdd(&sc->census_out, cens);
+    if (SURR) atomicAdd(&sc->surr_events, nsurr);
   }
   tally_flush<Acc, TM>(lds, NB, (Acc*)partial_v);
 }
@@ -710,23 +879,57 @@ static void pick_tally(const Ctx& c, int nbins, int* tally_mode, size_t* lds_byt
   else if (req == 3) { *tally_mode = TM_GLOBAL; *lds_bytes = 0; }
   else if (block_row <= 64 * 1024) { *tally_mode = TM_BLOCKROW; *lds_bytes = block_row; }
   else { *tally_mode = TM_GLOBAL; *lds_bytes = 0; }
+  *lds_bytes = (*lds_bytes + 15) & ~(size_t)15;
 }
+
+static bool use_surrogate(const Ctx& c) { return c.surr_enabled && !c.is2d; }
+
+// Dispatch over (precision, tally mode, geometry, surrogate).  F(P, TM, D, S) expands to
+// a statement using the kernel k_transport<P, TM, D, S>.
+#define IMC_TRANSPORT_DISPATCH(F)                                                                    \
+  do {                                                                                               \
+    const bool surr_ = use_surrogate(c);                                                             \
+    if (c.prec == 64) {                                                                              \
+      if (c.is2d) { if (tm == TM_WAVEROW) F(64, TM_WAVEROW, true, false);                            \
+                    else if (tm == TM_BLOCKROW) F(64, TM_BLOCKROW, true, false);                     \
+                    else F(64, TM_GLOBAL, true, false); }                                            \
+      else if (surr_) { if (tm == TM_GLOBAL) F(64, TM_GLOBAL, false, true);                          \
+                        else F(64, TM_BLOCKROW, false, true); }                                      \
+      else { if (tm == TM_WAVEROW) F(64, TM_WAVEROW, false, false);                                  \
+             else if (tm == TM_BLOCKROW) F(64, TM_BLOCKROW, false, false);                           \
+             else F(64, TM_GLOBAL, false, false); }                                                  \
+    } else if (c.prec == 32) {                                                                       \
+      if (c.is2d) { if (tm == TM_WAVEROW) F(32, TM_WAVEROW, true, false);                            \
+                    else if (tm == TM_BLOCKROW) F(32, TM_BLOCKROW, true, false);                     \
+                    else F(32, TM_GLOBAL, true, false); }                                            \
+      else if (surr_) { if (tm == TM_GLOBAL) F(32, TM_GLOBAL, false, true);                          \
+                        else F(32, TM_BLOCKROW, false, true); }                                      \
+      else { if (tm == TM_WAVEROW) F(32, TM_WAVEROW, false, false);                                  \
+             else if (tm == TM_BLOCKROW) F(32, TM_BLOCKROW, false, false);                           \
+             else F(32, TM_GLOBAL, false, false); }                                                  \
+    } else {                                                                                         \
+      if (c.is2d) { if (tm == TM_WAVEROW) F(16, TM_WAVEROW, true, false);                            \
+                    else if (tm == TM_BLOCKROW) F(16, TM_BLOCKROW, true, false);                     \
+                    else F(16, TM_GLOBAL, true, false); }                                            \
+      else if (surr_) { if (tm == TM_GLOBAL) F(16, TM_GLOBAL, false, true);                          \
+                        else F(16, TM_BLOCKROW, false, true); }                                      \
+      else { if (tm == TM_WAVEROW) F(16, TM_WAVEROW, false, false);                                  \
+             else if (tm == TM_BLOCKROW) F(16, TM_BLOCKROW, false, false);                           \
+             else F(16, TM_GLOBAL, false, false); }                                                  \
+    }                                                                                                \
+  } while (0)
 
 int transport_choose(const Ctx& c, int* grid_blocks, int* tally_mode, size_t* lds_bytes) {
   pick_tally(c, c.NB, tally_mode, lds_bytes);
+  if (use_surrogate(c) && *tally_mode == TM_WAVEROW) pick_tally(c, c.NB, tally_mode, lds_bytes);
+  if (use_surrogate(c) && *tally_mode == TM_WAVEROW) { *tally_mode = TM_GLOBAL; *lds_bytes = 0; }
+  const size_t total = *lds_bytes + (use_surrogate(c) ? SURR_LDS : 0);
+  const int tm = *tally_mode;
   int per_cu = 0;
-  hipError_t err;
-#define OCC(P, TM_, D)                                                                             \
-  err = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_transport<P, TM_, D>, TPB, *lds_bytes)
-#define OCCD(P, TM_) do { if (c.is2d) OCC(P, TM_, true); else OCC(P, TM_, false); } while (0)
-#define OCCT(P) do { if (*tally_mode == TM_WAVEROW) OCCD(P, TM_WAVEROW);                          \
-                     else if (*tally_mode == TM_BLOCKROW) OCCD(P, TM_BLOCKROW);                   \
-                     else OCCD(P, TM_GLOBAL); } while (0)
-  if (c.prec == 64) OCCT(64);
-  else if (c.prec == 32) OCCT(32);
-  else OCCT(16);
-#undef OCCT
-#undef OCCD
+  hipError_t err = hipSuccess;
+#define OCC(P, TM_, D, S) \
+  err = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_transport<P, TM_, D, S>, TPB, total)
+  IMC_TRANSPORT_DISPATCH(OCC);
 #undef OCC
   if (err != hipSuccess || per_cu <= 0) per_cu = 1;
   *grid_blocks = per_cu * c.num_cu;
@@ -736,17 +939,15 @@ int transport_choose(const Ctx& c, int* grid_blocks, int* tally_mode, size_t* ld
 hipError_t launch_transport(Ctx& c, int grid, int tm, size_t lds) {
   const Geo g = make_geo(c);
   SoA s = c.buf[c.cur];
-#define TR(P, TM_, D)                                                                            \
-  k_transport<P, TM_, D><<<grid, TPB, lds, c.stream>>>(s, g, c.d_sc, c.d_partial, c.d_dep)
-#define TRD(P, TM_) do { if (c.is2d) TR(P, TM_, true); else TR(P, TM_, false); } while (0)
-#define TRT(P) do { if (tm == TM_WAVEROW) TRD(P, TM_WAVEROW);                                     \
-                    else if (tm == TM_BLOCKROW) TRD(P, TM_BLOCKROW);                              \
-                    else TRD(P, TM_GLOBAL); } while (0)
-  if (c.prec == 64) TRT(64);
-  else if (c.prec == 32) TRT(32);
-  else TRT(16);
-#undef TRT
-#undef TRD
+  SurrArgs sa;
+  sa.wbf = c.d_surr_wbf;
+  sa.bias = c.d_surr_bias;
+  sa.box = c.surr_box;
+  sa.tally_bytes = (int)lds;
+  const size_t total = lds + (use_surrogate(c) ? SURR_LDS : 0);
+#define TR(P, TM_, D, S) \
+  k_transport<P, TM_, D, S><<<grid, TPB, total, c.stream>>>(s, g, c.d_sc, c.d_partial, c.d_dep, sa)
+  IMC_TRANSPORT_DISPATCH(TR);
 #undef TR
   return hipGetLastError();
 }

@@ -1,0 +1,176 @@
+// imc_mlp.h -- the thick-cell surrogate network 6 -> 64 -> 64 -> 4 on gfx950 matrix cores.
+//
+// All matrix products use v_mfma_f32_16x16x16bf16_1k (BF16 operands, FP32 accumulation).
+// Operand layouts of that instruction (16x16 tiles):
+//   A[i][k]: lane i + 16*(k/4), element k%4      B[k][j]: lane j + 16*(k/4), element k%4
+//   D[i][j]: lane j + 16*(i/4), element i%4
+// A wave evaluates the network for up to 16 samples per pass (one 16-row tile), so a
+// wave only pays for the 16-lane groups that hold samples.
+//
+// Parameter vector (FP32 master copy, 4868 values):
+//   W1[6][64] (in x out), b1[64], W2[64][64], b2[64], W3[64][4], b3[4]
+// Inference copy (BF16, transposed, zero padded): W1t[64][16], W2t[64][64], W3t[16][64]
+// plus FP32 biases b1[64], b2[64], b3[4].
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "imc_internal.h"
+
+namespace imc {
+
+constexpr int MLP_IN = 6, MLP_H = 64, MLP_OUT = 4;
+constexpr int MLP_NPARAM = MLP_IN * MLP_H + MLP_H + MLP_H * MLP_H + MLP_H + MLP_H * MLP_OUT + MLP_OUT;  // 4868
+constexpr int OFF_W1 = 0, OFF_B1 = OFF_W1 + MLP_IN * MLP_H, OFF_W2 = OFF_B1 + MLP_H,
+              OFF_B2 = OFF_W2 + MLP_H * MLP_H, OFF_W3 = OFF_B2 + MLP_H, OFF_B3 = OFF_W3 + MLP_H * MLP_OUT;
+// bf16 inference block (uint16 units): W1t 64x16, W2t 64x64, W3t 16x64
+constexpr int BF_W1T = 0, BF_W2T = BF_W1T + 64 * 16, BF_W3T = BF_W2T + 64 * 64, BF_TOTAL = BF_W3T + 16 * 64;
+constexpr int BIAS_TOTAL = 64 + 64 + 4;
+
+typedef short mshort4 __attribute__((ext_vector_type(4)));
+typedef float mfloat4 __attribute__((ext_vector_type(4)));
+
+// float -> bf16, round to nearest even (inputs are finite).
+__device__ __forceinline__ uint16_t f2bf(float f) {
+  const uint32_t u = __float_as_uint(f);
+  return (uint16_t)((u + 0x7FFFu + ((u >> 16) & 1u)) >> 16);
+}
+__device__ __forceinline__ float bf2f(uint16_t h) { return __uint_as_float(((uint32_t)h) << 16); }
+
+__device__ __forceinline__ mfloat4 mfma_bf16(mshort4 a, mshort4 b, mfloat4 c) {
+  return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
+}
+
+// Fragment of a 16x16 operand tile: element j of lane l is
+// base[(i0 + l%16) * si + (k0 + 4*(l/16) + j) * sk]  (A: i = row; B: i = column).
+__device__ __forceinline__ mshort4 frag(const uint16_t* base, int si, int sk, int i0, int k0) {
+  const int l = (int)(threadIdx.x & 63u);
+  const int i = i0 + (l & 15), k = k0 + 4 * (l >> 4);
+  mshort4 v;
+  v[0] = (short)base[i * si + (k + 0) * sk];
+  v[1] = (short)base[i * si + (k + 1) * sk];
+  v[2] = (short)base[i * si + (k + 2) * sk];
+  v[3] = (short)base[i * si + (k + 3) * sk];
+  return v;
+}
+
+// Orders this wave's LDS traffic: writes by one lane before reads by another lane.
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+__device__ __forceinline__ float sigmoidf_(float z) { return 1.0f / (1.0f + __expf(-z)); }
+
+// Per-wave LDS scratch for one 16-sample pass: xs[16][16], h1[16][64], h2[16][64] (bf16)
+// and y[16][4] (fp32).
+struct MlpScratch {
+  uint16_t xs[16 * 16];
+  uint16_t h1[16 * 64];
+  uint16_t h2[16 * 64];
+  float y[16 * 4];
+};
+
+// Evaluates the network for the lanes with `active` set (all 64 lanes must call).
+// feat[6] are the lane's normalised inputs; out[4] receives the sigmoid outputs.
+__device__ __forceinline__ void mlp_forward_wave(const uint16_t* __restrict__ wbf,   // BF_TOTAL
+                                                 const float* __restrict__ bias,      // BIAS_TOTAL
+                                                 MlpScratch* sc, bool active, const float (&feat)[MLP_IN],
+                                                 float (&out)[MLP_OUT]) {
+  const int l = (int)(threadIdx.x & 63u);
+  const uint64_t mask = __ballot(active);
+  const uint16_t* W1t = wbf + BF_W1T;
+  const uint16_t* W2t = wbf + BF_W2T;
+  const uint16_t* W3t = wbf + BF_W3T;
+  const float* b1 = bias;
+  const float* b2 = bias + 64;
+  const float* b3 = bias + 128;
+  for (int c = 0; c < 4; ++c) {
+    if (((mask >> (16 * c)) & 0xFFFFull) == 0ull) continue;   // wave-uniform
+    // stage the 16 samples of this group (zero rows for idle lanes, zero padding cols)
+    if ((l >> 4) == c) {
+      const int r = l & 15;
+#pragma unroll
+      for (int j = 0; j < 16; ++j) sc->xs[r * 16 + j] = (active && j < MLP_IN) ? f2bf(feat[j]) : (uint16_t)0;
+    }
+    wave_lds_sync();
+    // layer 1: h1 = relu(x W1 + b1)
+#pragma unroll
+    for (int nt = 0; nt < 4; ++nt) {
+      mfloat4 acc = {0.f, 0.f, 0.f, 0.f};
+      acc = mfma_bf16(frag(sc->xs, 16, 1, 0, 0), frag(W1t, 16, 1, 16 * nt, 0), acc);
+      const int col = 16 * nt + (l & 15);
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int row = 4 * (l >> 4) + r;
+        sc->h1[row * 64 + col] = f2bf(fmaxf(acc[r] + b1[col], 0.f));
+      }
+    }
+    wave_lds_sync();
+    // layer 2: h2 = relu(h1 W2 + b2)
+#pragma unroll
+    for (int nt = 0; nt < 4; ++nt) {
+      mfloat4 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+      for (int kt = 0; kt < 4; ++kt)
+        acc = mfma_bf16(frag(sc->h1, 64, 1, 0, 16 * kt), frag(W2t, 64, 1, 16 * nt, 16 * kt), acc);
+      const int col = 16 * nt + (l & 15);
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int row = 4 * (l >> 4) + r;
+        sc->h2[row * 64 + col] = f2bf(fmaxf(acc[r] + b2[col], 0.f));
+      }
+    }
+    wave_lds_sync();
+    // layer 3: y = h2 W3 + b3 (4 of the 16 output columns are used)
+    {
+      mfloat4 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+      for (int kt = 0; kt < 4; ++kt)
+        acc = mfma_bf16(frag(sc->h2, 64, 1, 0, 16 * kt), frag(W3t, 64, 1, 0, 16 * kt), acc);
+      const int col = l & 15;
+      if (col < MLP_OUT) {
+#pragma unroll
+        for (int r = 0; r < 4; ++r) sc->y[(4 * (l >> 4) + r) * 4 + col] = acc[r] + b3[col];
+      }
+    }
+    wave_lds_sync();
+    if ((l >> 4) == c && active) {
+      const int r = l & 15;
+#pragma unroll
+      for (int o = 0; o < MLP_OUT; ++o) out[o] = sigmoidf_(sc->y[r * 4 + o]);
+    }
+    wave_lds_sync();
+  }
+}
+
+// ---------------------------------------------------------------------------------
+// Feature map of the single-cell problem.  All lengths in units of the collision mean
+// free path 1/sigma_c (sigma_c = sigma_a(1-f) + sigma_s):
+//   tau_cell = sigma_c * L, xi = x/L, mu, tau_B = sigma_c * min(census path, cut-off path),
+//   kappa = f sigma_a / sigma_c (weight decay per mean free path).
+// ---------------------------------------------------------------------------------
+__device__ __forceinline__ float map_log(float v, float lo, float hi) {
+  return 2.0f * (__logf(v) - __logf(lo)) / (__logf(hi) - __logf(lo)) - 1.0f;
+}
+
+__device__ __forceinline__ void surr_features(const SurrBox& bx, float tau_cell, float xi, float mu,
+                                              float tau_b, float kappa, float (&f)[MLP_IN]) {
+  f[0] = map_log(tau_cell, bx.tau_cell_min, bx.tau_cell_max);
+  f[1] = 2.0f * xi - 1.0f;
+  f[2] = mu;
+  f[3] = map_log(tau_b, bx.tau_b_min, bx.tau_b_max);
+  f[4] = map_log(kappa, bx.kappa_min, bx.kappa_max);
+  // diffusion-time ratio tau_B / tau_cell^2 over its range in the box
+  const float lo = bx.tau_b_min / (bx.tau_cell_max * bx.tau_cell_max);
+  const float hi = bx.tau_b_max / (bx.tau_cell_min * bx.tau_cell_min);
+  f[5] = map_log(tau_b / (tau_cell * tau_cell), lo, hi);
+}
+
+__device__ __forceinline__ bool surr_in_box(const SurrBox& bx, float tau_cell, float tau_b, float kappa) {
+  return tau_cell >= bx.tau_cell_min && tau_cell <= bx.tau_cell_max && tau_b >= bx.tau_b_min &&
+         tau_b <= bx.tau_b_max && kappa >= bx.kappa_min && kappa <= bx.kappa_max;
+}
+
+}  // namespace imc
