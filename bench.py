@@ -25,6 +25,7 @@ PKG = os.path.join(ROOT, "mixedprecisionimc.jl_amd")
 sys.path.insert(0, PKG)
 
 DECK = os.path.join(ROOT, "decks", "c2_marshak_fp32.txt")
+DECK_SURR = os.path.join(ROOT, "decks", "c4_thickthin_fp32.txt")
 HBM_PEAK_GBS = 8000.0
 
 
@@ -64,6 +65,74 @@ def cpu_baseline(inputs, mesh, steps: int):
                                 "speedup_hot_path": g_hot / cpu}}
 
 
+def _c4(prec, n_input):
+    from MixedPrecisionIMC import Input
+
+    inp = Input.readInputs(DECK_SURR)
+    inp["PRECISION"] = prec
+    inp["NINPUT"] = str(n_input)
+    inp["NMAX"] = str(10 * n_input)
+    return inp
+
+
+def _run_c4(prec, n_input, seed, steps, params=None, train=False):
+    import MixedPrecisionIMC as M
+    from MixedPrecisionIMC import Surrogate
+
+    inp = _c4(prec, n_input)
+    mesh, simvars, particles = M.setup(inp, seed=seed)
+    info = None
+    if train:
+        info = Surrogate.train(particles, seed=seed)
+        params = particles.surrogate_params()
+    elif params is not None:
+        Surrogate.load(particles, params)
+    particles.reset_timing()
+    for _ in range(steps):
+        M.step(inp, mesh, simvars, particles, "CONSTANT")
+    tm = particles.timing()
+    particles.close()
+    hot_s = (tm["ms_source"] + tm["ms_transport"] + tm["ms_clean"] + tm["ms_tally"]) / 1e3
+    return mesh.temp.copy(), hot_s, tm, params, info
+
+
+def surrogate_leg(steps=20, n_input=400000, seeds=4):
+    """Thick/thin blocks workload: FP32 plain vs FP32 + learned surrogate at equal tally
+    error against an FP64 GPU oracle (plain FP64, 4x the particles, 4 seeds).
+    eps = relative RMS over cells of the final material temperature; FOM = 1/(eps^2 t)."""
+    import numpy as np
+
+    ref = np.mean([_run_c4(64, 4 * n_input, 9000 + s, steps)[0] for s in range(seeds)], axis=0)
+    out = {}
+    params = None
+    train_info = None
+    for label in ("plain", "surrogate"):
+        eps2, t, hist, segs, nsurr = [], 0.0, 0, 0, 0
+        for s in range(seeds):
+            temp, hot, tm, params_out, info = _run_c4(32, n_input, 100 * (label == "surrogate") + s, steps,
+                                                      params=params if label == "surrogate" else None,
+                                                      train=(label == "surrogate" and params is None))
+            if label == "surrogate" and params is None:
+                params, train_info = params_out, info
+            eps2.append(float(np.mean(((temp - ref) / ref) ** 2)))
+            t += hot
+            hist += tm["histories"]
+            segs += tm["segments"]
+            nsurr += tm["surrogate_events"]
+        e2 = float(np.mean(eps2))
+        out[label] = {"eps_rms": e2 ** 0.5, "gpu_s_per_run": t / seeds, "histories_per_s": hist / t,
+                      "segments_per_run": segs / seeds, "surrogate_events_per_run": nsurr / seeds,
+                      "fom": 1.0 / (e2 * t / seeds)}
+    out["fom_speedup"] = out["surrogate"]["fom"] / out["plain"]["fom"]
+    out["time_speedup"] = out["plain"]["gpu_s_per_run"] / out["surrogate"]["gpu_s_per_run"]
+    out["workload"] = (f"thick/thin blocks 1D (decks/c4_thickthin_fp32.txt), {steps} steps, NINPUT {n_input}, "
+                       f"{seeds} seeds each; oracle: FP64 plain, 4x particles, {seeds} seeds")
+    if train_info:
+        out["training"] = {k: train_info[k] for k in ("samples", "events", "generate_s", "train_s")}
+        out["training"]["final_loss"] = train_info["loss_per_epoch"][-1]
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -72,6 +141,7 @@ def main():
     ap.add_argument("--deck", default=DECK)
     ap.add_argument("--cpu-steps", type=int, default=12)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-surrogate", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the engine runs on a "
@@ -133,9 +203,13 @@ def main():
     }
     print(f"[bench] {done} steps, {hist} histories, {tm['segments']} segments, wall {wall:.3f}s, "
           f"transport kernel {tm['ms_transport_kernel']:.1f} ms over {launches} launches", file=sys.stderr, flush=True)
+    particles.close()
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(inputs, mesh, args.cpu_steps)
-    particles.close()
+    if not args.no_surrogate:
+        line["surrogate"] = surrogate_leg()
+        print(f"[bench] surrogate leg: FOM speedup {line['surrogate']['fom_speedup']:.2f}", file=sys.stderr,
+              flush=True)
     print(json.dumps(line))
 
 

@@ -13,7 +13,7 @@ import time
 
 import numpy as np
 
-DEFAULT_BOX = {"tau_cell": (8.0, 200.0), "tau_b": (2.0, 1.0e4), "kappa": (1.0e-5, 4.0), "depth_min": 2.5}
+DEFAULT_BOX = {"tau_cell": (5.0, 200.0), "tau_b": (2.0, 1.0e4), "kappa": (1.0e-5, 4.0), "depth_min": 2.0}
 
 
 def train(particles, *, n_samples: int = 10_000_000, epochs: int = 12, batch: int = 8192, lr: float = 3e-3,

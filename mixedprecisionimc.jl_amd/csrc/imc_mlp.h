@@ -4,8 +4,8 @@
 // Operand layouts of that instruction (16x16 tiles):
 //   A[i][k]: lane i + 16*(k/4), element k%4      B[k][j]: lane j + 16*(k/4), element k%4
 //   D[i][j]: lane j + 16*(i/4), element i%4
-// A wave evaluates the network for up to 16 samples per pass (one 16-row tile), so a
-// wave only pays for the 16-lane groups that hold samples.
+// A wave evaluates the network for up to 16 samples per pass (one 16-column tile), so
+// a wave only pays for the 16-lane groups that hold samples.
 //
 // Parameter vector (FP32 master copy, 4868 values):
 //   W1[6][64] (in x out), b1[64], W2[64][64], b2[64], W3[64][4], b3[4]
@@ -54,6 +54,12 @@ __device__ __forceinline__ mshort4 frag(const uint16_t* base, int si, int sk, in
   return v;
 }
 
+// Contiguous fragment (sk == 1, row stride a multiple of 4): one 8-byte load.
+__device__ __forceinline__ mshort4 frag_c(const uint16_t* base, int si, int i0, int k0) {
+  const int l = (int)(threadIdx.x & 63u);
+  return *(const mshort4*)(base + (i0 + (l & 15)) * si + k0 + 4 * (l >> 4));
+}
+
 // Orders this wave's LDS traffic: writes by one lane before reads by another lane.
 __device__ __forceinline__ void wave_lds_sync() {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -63,22 +69,27 @@ __device__ __forceinline__ void wave_lds_sync() {
 
 __device__ __forceinline__ float sigmoidf_(float z) { return 1.0f / (1.0f + __expf(-z)); }
 
-// Per-wave LDS scratch for one 16-sample pass: xs[16][16], h1[16][64], h2[16][64] (bf16)
-// and y[16][4] (fp32).
+// Per-wave LDS scratch of one 16-sample pass: the input tile xs[16 samples][16] (bf16)
+// and the outputs y[16 samples][4] (fp32).
 struct MlpScratch {
   uint16_t xs[16 * 16];
-  uint16_t h1[16 * 64];
-  uint16_t h2[16 * 64];
   float y[16 * 4];
 };
 
 // Evaluates the network for the lanes with `active` set (all 64 lanes must call).
 // feat[6] are the lane's normalised inputs; out[4] receives the sigmoid outputs.
+//
+// Samples are the columns: H1^T = relu(W1^T X^T + b1), H2^T = relu(W2^T H1^T + b2),
+// Y^T = W3^T H2^T + b3.  With the 16x16x16 layouts above, the D tile of one layer
+// (lane holds rows 4*(l/16)..+3 of column l%16) is exactly the B fragment that the next
+// layer needs for the same 16 rows as its k range, so hidden activations never leave the
+// registers; the weights are the A operands (8-byte LDS loads).
 __device__ __forceinline__ void mlp_forward_wave(const uint16_t* __restrict__ wbf,   // BF_TOTAL
                                                  const float* __restrict__ bias,      // BIAS_TOTAL
                                                  MlpScratch* sc, bool active, const float (&feat)[MLP_IN],
                                                  float (&out)[MLP_OUT]) {
   const int l = (int)(threadIdx.x & 63u);
+  const int grp = l >> 4;
   const uint64_t mask = __ballot(active);
   const uint16_t* W1t = wbf + BF_W1T;
   const uint16_t* W2t = wbf + BF_W2T;
@@ -88,58 +99,43 @@ __device__ __forceinline__ void mlp_forward_wave(const uint16_t* __restrict__ wb
   const float* b3 = bias + 128;
   for (int c = 0; c < 4; ++c) {
     if (((mask >> (16 * c)) & 0xFFFFull) == 0ull) continue;   // wave-uniform
-    // stage the 16 samples of this group (zero rows for idle lanes, zero padding cols)
-    if ((l >> 4) == c) {
-      const int r = l & 15;
+    if (grp == c) {
+      const int sidx = l & 15;
 #pragma unroll
-      for (int j = 0; j < 16; ++j) sc->xs[r * 16 + j] = (active && j < MLP_IN) ? f2bf(feat[j]) : (uint16_t)0;
+      for (int j = 0; j < 16; ++j) sc->xs[sidx * 16 + j] = (active && j < MLP_IN) ? f2bf(feat[j]) : (uint16_t)0;
     }
     wave_lds_sync();
-    // layer 1: h1 = relu(x W1 + b1)
+    const mshort4 bx = frag_c(sc->xs, 16, 0, 0);             // B[k][s] = X[s][k]
+    mshort4 h1[4], h2[4];
 #pragma unroll
-    for (int nt = 0; nt < 4; ++nt) {
+    for (int mt = 0; mt < 4; ++mt) {                          // layer 1
       mfloat4 acc = {0.f, 0.f, 0.f, 0.f};
-      acc = mfma_bf16(frag(sc->xs, 16, 1, 0, 0), frag(W1t, 16, 1, 16 * nt, 0), acc);
-      const int col = 16 * nt + (l & 15);
+      acc = mfma_bf16(frag_c(W1t, 16, 16 * mt, 0), bx, acc);
 #pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const int row = 4 * (l >> 4) + r;
-        sc->h1[row * 64 + col] = f2bf(fmaxf(acc[r] + b1[col], 0.f));
-      }
+      for (int r = 0; r < 4; ++r)
+        h1[mt][r] = (short)f2bf(fmaxf(acc[r] + b1[16 * mt + 4 * grp + r], 0.f));
     }
-    wave_lds_sync();
-    // layer 2: h2 = relu(h1 W2 + b2)
 #pragma unroll
-    for (int nt = 0; nt < 4; ++nt) {
+    for (int mt = 0; mt < 4; ++mt) {                          // layer 2
       mfloat4 acc = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-      for (int kt = 0; kt < 4; ++kt)
-        acc = mfma_bf16(frag(sc->h1, 64, 1, 0, 16 * kt), frag(W2t, 64, 1, 16 * nt, 16 * kt), acc);
-      const int col = 16 * nt + (l & 15);
+      for (int kt = 0; kt < 4; ++kt) acc = mfma_bf16(frag_c(W2t, 64, 16 * mt, 16 * kt), h1[kt], acc);
 #pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const int row = 4 * (l >> 4) + r;
-        sc->h2[row * 64 + col] = f2bf(fmaxf(acc[r] + b2[col], 0.f));
-      }
+      for (int r = 0; r < 4; ++r)
+        h2[mt][r] = (short)f2bf(fmaxf(acc[r] + b2[16 * mt + 4 * grp + r], 0.f));
+    }
+    mfloat4 acc = {0.f, 0.f, 0.f, 0.f};                       // layer 3: rows 0..3 are outputs
+#pragma unroll
+    for (int kt = 0; kt < 4; ++kt) acc = mfma_bf16(frag_c(W3t, 64, 0, 16 * kt), h2[kt], acc);
+    if (grp == 0) {
+#pragma unroll
+      for (int r = 0; r < 4; ++r) sc->y[(l & 15) * 4 + r] = acc[r] + b3[r];
     }
     wave_lds_sync();
-    // layer 3: y = h2 W3 + b3 (4 of the 16 output columns are used)
-    {
-      mfloat4 acc = {0.f, 0.f, 0.f, 0.f};
+    if (grp == c && active) {
+      const int sidx = l & 15;
 #pragma unroll
-      for (int kt = 0; kt < 4; ++kt)
-        acc = mfma_bf16(frag(sc->h2, 64, 1, 0, 16 * kt), frag(W3t, 64, 1, 0, 16 * kt), acc);
-      const int col = l & 15;
-      if (col < MLP_OUT) {
-#pragma unroll
-        for (int r = 0; r < 4; ++r) sc->y[(4 * (l >> 4) + r) * 4 + col] = acc[r] + b3[col];
-      }
-    }
-    wave_lds_sync();
-    if ((l >> 4) == c && active) {
-      const int r = l & 15;
-#pragma unroll
-      for (int o = 0; o < MLP_OUT; ++o) out[o] = sigmoidf_(sc->y[r * 4 + o]);
+      for (int o = 0; o < MLP_OUT; ++o) out[o] = sigmoidf_(sc->y[sidx * 4 + o]);
     }
     wave_lds_sync();
   }
