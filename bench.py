@@ -26,43 +26,103 @@ sys.path.insert(0, PKG)
 
 DECK = os.path.join(ROOT, "decks", "c2_marshak_fp32.txt")
 DECK_SURR = os.path.join(ROOT, "decks", "c4_thickthin_fp32.txt")
+DECK_SLAB = os.path.join(ROOT, "decks", "c1_slab_fp64.txt")
 HBM_PEAK_GBS = 8000.0
 
 
 TALLY_MODES = {0: "WAVEROW", 1: "BLOCKROW", 2: "GLOBAL"}
 
 
-def gpu_window(inputs, steps: int):
-    """GPU rate on the same sample as the CPU baseline: the first `steps` timesteps."""
-    import MixedPrecisionIMC as M
-
-    mesh, simvars, particles = M.setup(inputs)
-    ts = inputs["TIMESTEPPING"].upper() if isinstance(inputs["TIMESTEPPING"], str) else "RAMP"
-    particles.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(steps):
-        M.step(inputs, mesh, simvars, particles, ts)
-    particles.synchronize()
-    wall = time.perf_counter() - t0
-    tm = particles.timing()
-    particles.close()
-    hot = (tm["ms_source"] + tm["ms_transport"] + tm["ms_clean"] + tm["ms_tally"]) / 1e3
-    return tm["histories"] / wall, tm["histories"] / hot
+def profiled_traffic():
+    """HBM bytes per transport launch on this workload, from the committed rocprofv3 counter
+    summary (FETCH_SIZE and WRITE_SIZE passes, scaled by a plain copy's readings)."""
+    path = os.path.join(ROOT, "profiles", "transport_marshak_hbm_traffic.md")
+    try:
+        with open(path) as f:
+            for ln in f:
+                if ln.startswith("traffic_bytes_per_launch:"):
+                    return float(ln.split(":", 1)[1]), os.path.relpath(path, ROOT)
+    except OSError:
+        pass
+    return None, None
 
 
-def cpu_baseline(inputs, mesh, steps: int):
+def _hot_s(tm) -> float:
+    """GPU time of the particle loop (sourcing + transport + clean + census tally), seconds."""
+    return (tm["ms_source"] + tm["ms_transport"] + tm["ms_clean"] + tm["ms_tally"]) / 1e3
+
+
+def _oracle():
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O  # the checker, timed here as the CPU baseline only
+    return O
 
-    threads = min(16, os.cpu_count() or 1)
-    r = O.run_deck(inputs, mesh, max_steps=steps, nthreads=threads)
-    g_wall, g_hot = gpu_window(inputs, steps)
-    cpu = r["histories"] / r["t_hot"]
-    return {"value": cpu, "unit": "histories/s", "cores": threads, "kind": "port",
-            "sample": f"first {r['steps']} timesteps of the same deck from t=0 ({r['histories']} histories, "
-                      f"C++ FP32 oracle, OpenMP over particles), source+transport+clean+census time",
-            "gpu_same_sample": {"value_wall": g_wall, "value_hot_path": g_hot,
-                                "speedup_hot_path": g_hot / cpu}}
+
+def _threads() -> int:
+    return min(16, os.cpu_count() or 1)
+
+
+def oracle_window(inputs, mesh, warmup: int, steps: int, prec=None):
+    """CPU rate of the oracle on timesteps warmup .. warmup+steps-1 of a deck (the earlier
+    steps run untimed, so the sample has the same particle population as the GPU window)."""
+    r = _oracle().run_deck(inputs, mesh, prec=prec, max_steps=warmup + steps, time_from=warmup,
+                           nthreads=_threads())
+    return r["histories"] / r["t_hot"], r
+
+
+def gpu_run(inputs, warmup: int, steps: int, precision=None):
+    """GPU hot-path rate on timesteps warmup .. warmup+steps-1 of a deck."""
+    import MixedPrecisionIMC as M
+
+    mesh, simvars, particles = M.setup(inputs, precision=precision)
+    ts = str(inputs["TIMESTEPPING"]).upper()
+    for _ in range(warmup):
+        M.step(inputs, mesh, simvars, particles, ts)
+    particles.synchronize()
+    particles.reset_timing()
+    done = 0
+    while done < steps and simvars.t <= simvars.t_end:
+        M.step(inputs, mesh, simvars, particles, ts)
+        done += 1
+    particles.synchronize()
+    tm = particles.timing()
+    particles.close()
+    return tm["histories"] / _hot_s(tm), tm, done
+
+
+def cpu_baseline(inputs, mesh, warmup: int, steps: int, gpu_same):
+    cpu, r = oracle_window(inputs, mesh, warmup, steps)
+    if gpu_same is None:                       # the timed window was shorter than the sample
+        rate, tm, _ = gpu_run(inputs, warmup, steps)
+        gpu_same = (tm["histories"], tm["histories"] / rate)
+    g_hist, g_hot = gpu_same
+    return {"value": cpu, "unit": "histories/s", "cores": _threads(), "kind": "port",
+            "sample": f"timesteps {warmup}..{warmup + steps - 1} of the same deck, i.e. the first {steps} "
+                      f"steps of the GPU's timed window "
+                      f"({r['histories']} histories, {r['segments']} segments; {warmup} earlier steps run untimed); "
+                      f"C++ FP32 oracle, OpenMP over particles; sourcing+transport+clean+census time",
+            "gpu_same_sample": {"histories_per_s_hot_path": g_hist / g_hot,
+                                "speedup_hot_path": (g_hist / g_hot) / cpu}}
+
+
+def fp64_leg(warmup: int, steps: int):
+    """Plain FP64 tracking on the GPU against the FP64 oracle on the host cores: the Marshak
+    deck at FLOAT64 on the same step window, and the reference's CPU slab case (400 cells,
+    linearized, NINPUT 1e5, 100 steps) run to the end."""
+    from MixedPrecisionIMC import Input, Mesh, Constants
+
+    out = {}
+    for name, deck, w, k in (("marshak_fp64", DECK, warmup, steps), ("slab_fp64", DECK_SLAB, 0, 1000)):
+        inp = Input.readInputs(deck)
+        inp["PRECISION"] = "64"
+        g, tm, done = gpu_run(inp, w, k, precision=64)
+        Constants.set_constants(inp)
+        mesh = Mesh.mesh_generation(inp)
+        c, r = oracle_window(inp, mesh, w, done, prec=64)
+        out[name] = {"gpu_histories_per_s": g, "cpu_histories_per_s": c, "speedup": g / c, "cores": _threads(),
+                     "steps": f"{w}..{w + done - 1}", "gpu_histories": tm["histories"],
+                     "cpu_histories": r["histories"]}
+    return out
 
 
 def _c4(prec, n_input):
@@ -160,11 +220,15 @@ def main():
     particles.reset_timing()
     t0 = time.perf_counter()
     done = 0
+    snap = None
     for _ in range(args.steps):
         if simvars.t > simvars.t_end:
             break
         M.step(inputs, mesh, simvars, particles, timestepping)
         done += 1
+        if done == args.cpu_steps and not args.no_cpu:
+            t_snap = particles.timing()      # GPU figures on the CPU baseline's sample
+            snap = (t_snap["histories"], _hot_s(t_snap))
     particles.synchronize()
     wall = time.perf_counter() - t0
     tm = particles.timing()
@@ -174,6 +238,7 @@ def main():
     launches = max(tm["n_transport_launches"], 1)
     alg_bytes = sbytes * (hist + tm["census_out"])
     achieved = alg_bytes / kern_s / 1e9 if kern_s > 0 else 0.0
+    traffic, traffic_src = profiled_traffic()
     line = {
         "metric": "histories/sec at equal tally error vs FP64 oracle; surrogate vs plain speedup",
         "value": hist / wall,
@@ -193,19 +258,24 @@ def main():
                    "histories": hist, "segments": tm["segments"], "census_out": tm["census_out"],
                    "state_bytes": sbytes},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "traffic_source": traffic_src,
                      "kernel": f"k_transport<32,{TALLY_MODES.get(tm['transport_tally_mode'])},1D>",
                      "grid_blocks": tm["transport_grid"],
                      "avg_launch_ms": tm["ms_transport_kernel"] / launches,
-                     "bytes_per_launch": alg_bytes / launches},
+                     "bytes_per_launch": alg_bytes / launches,
+                     "segments_per_s": tm["segments"] / kern_s if kern_s > 0 else 0.0},
         "gpu_time_ms": {k: tm[k] for k in ("ms_source", "ms_transport", "ms_clean", "ms_tally",
                                            "ms_transport_kernel")},
+        "hot_path_histories_per_s": hist / _hot_s(tm),
     }
     print(f"[bench] {done} steps, {hist} histories, {tm['segments']} segments, wall {wall:.3f}s, "
           f"transport kernel {tm['ms_transport_kernel']:.1f} ms over {launches} launches", file=sys.stderr, flush=True)
     particles.close()
     if not args.no_cpu:
-        line["cpu_baseline"] = cpu_baseline(inputs, mesh, args.cpu_steps)
+        line["cpu_baseline"] = cpu_baseline(inputs, mesh, args.warmup, args.cpu_steps, snap)
+        print(f"[bench] cpu baseline {line['cpu_baseline']['value']:.4g} histories/s", file=sys.stderr, flush=True)
+        line["fp64"] = fp64_leg(args.warmup, args.cpu_steps)
+        print(f"[bench] fp64 leg done", file=sys.stderr, flush=True)
     if not args.no_surrogate:
         line["surrogate"] = surrogate_leg()
         print(f"[bench] surrogate leg: FOM speedup {line['surrogate']['fom_speedup']:.2f}", file=sys.stderr,

@@ -131,6 +131,9 @@ Geo make_geo(const Ctx& c) {
   g.dt = c.dt; g.c = c.cfg.phys_c; g.s = c.cfg.distancescale;
   g.dxs = c.d_dxs; g.dys = c.d_dys; g.dxw = c.d_dxw; g.dyw = c.d_dyw;
   g.sigcol = c.d_sigcol; g.sigdep = c.d_sigdep;
+  g.isig = c.d_isig; g.idxw = c.d_idxw; g.idyw = c.d_idyw;
+  g.ics = 1.0 / (c.cfg.phys_c * c.cfg.distancescale);
+  g.cell_off = -1;
   for (int k = 0; k < 16; ++k) g.scales[k] = c.cfg.energyscales[k];
   return g;
 }
@@ -186,6 +189,9 @@ int imc_create(imc_ctx** out, const imc_config* cfg) {
   ok &= dalloc(&c.d_tsurf, 8 * (c.is2d ? 2 * nx + 2 * ny : 2)) == hipSuccess;
   ok &= dalloc(&c.d_sigcol, T * C) == hipSuccess;
   ok &= dalloc(&c.d_sigdep, W * C) == hipSuccess;
+  ok &= dalloc(&c.d_isig, T * C) == hipSuccess;
+  ok &= dalloc(&c.d_idxw, W * nx) == hipSuccess;
+  ok &= dalloc(&c.d_idyw, W * ny) == hipSuccess;
   const size_t NE = (size_t)c.NE;
   ok &= dalloc(&c.d_ent_e, 8 * NE) == hipSuccess;
   ok &= dalloc(&c.d_ent_esc, 8 * NE) == hipSuccess;
@@ -224,7 +230,7 @@ int imc_destroy(imc_ctx* h) {
   if (c.stream) (void)hipStreamSynchronize(c.stream);
   free_soa(c.buf[0]); free_soa(c.buf[1]);
   void* ptrs[] = {c.d_dx, c.d_dy, c.d_dxs, c.d_dys, c.d_dxw, c.d_dyw, c.d_temp, c.d_fleck, c.d_sa, c.d_ss,
-                  c.d_radsrc, c.d_tsurf, c.d_sigcol, c.d_sigdep, c.d_ent_e, c.d_ent_esc, c.d_ent_term,
+                  c.d_radsrc, c.d_tsurf, c.d_sigcol, c.d_sigdep, c.d_isig, c.d_idxw, c.d_idyw, c.d_ent_e, c.d_ent_esc, c.d_ent_term,
                   c.d_ent_k, c.d_ent_cnt, c.d_ent_off, c.d_red, c.d_emitted, c.d_dep, c.d_rad, c.d_sc,
                   c.d_partial, c.d_tile, c.d_surr_feat, c.d_surr_targ, c.d_surr_params, c.d_surr_grad,
                   c.d_surr_m, c.d_surr_v, c.d_surr_wbf, c.d_surr_bias, c.d_surr_loss, c.d_surr_events};

@@ -24,10 +24,10 @@ __device__ __forceinline__ u4 philox(uint32_t c0, uint32_t c1, uint32_t c2, uint
 #pragma unroll
   for (int round = 0; round < 10; ++round) {
     if (round) { k0 += 0x9E3779B9u; k1 += 0xBB67AE85u; }
-    const uint32_t hi0 = __umulhi(0xD2511F53u, c0), lo0 = 0xD2511F53u * c0;
-    const uint32_t hi1 = __umulhi(0xCD9E8D57u, c2), lo1 = 0xCD9E8D57u * c2;
-    const uint32_t n0 = hi1 ^ c1 ^ k0, n2 = hi0 ^ c3 ^ k1;
-    c0 = n0; c1 = lo1; c2 = n2; c3 = lo0;
+    // one 32x32->64 product per word (v_mad_u64_u32) instead of separate hi and lo halves
+    const uint64_t p0 = (uint64_t)c0 * 0xD2511F53ull, p1 = (uint64_t)c2 * 0xCD9E8D57ull;
+    const uint32_t n0 = (uint32_t)(p1 >> 32) ^ c1 ^ k0, n2 = (uint32_t)(p0 >> 32) ^ c3 ^ k1;
+    c0 = n0; c1 = (uint32_t)p1; c2 = n2; c3 = (uint32_t)p0;
   }
   u4 o; o.r[0] = c0; o.r[1] = c1; o.r[2] = c2; o.r[3] = c3;
   return o;
@@ -133,9 +133,8 @@ template <> struct Tm<_Float16> {
 
 // Julia's min propagates NaN.
 template <class T> __device__ __forceinline__ T jmin(T a, T b) {
-  if (Tm<T>::nan_(a)) return a;
-  if (Tm<T>::nan_(b)) return b;
-  return (b < a) ? b : a;
+  const T m = (b < a) ? b : a;
+  return Tm<T>::nan_(a) ? a : (Tm<T>::nan_(b) ? b : m);      // selects, no branches
 }
 
 // Birth uniforms: double -> 53-bit pairs (two per Philox block), float -> 24-bit words.

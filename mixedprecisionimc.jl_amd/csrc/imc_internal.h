@@ -53,6 +53,11 @@ struct Geo {                   // kernel view of the geometry and step coefficie
   const void* dyw;             // W: dy
   const void* sigcol;          // T: sigma_a(1-f)+sigma_s per cell
   const void* sigdep;          // W: sigma_a*f per cell
+  const void* isig;            // T: 1/(sigma_a(1-f)+sigma_s) per cell
+  const void* idxw;            // W: 1/dx per column
+  const void* idyw;            // W: 1/dy per row (2D)
+  double ics;                  // 1/(phys_c * distancescale)
+  int cell_off;                // 1D: byte offset of the LDS cell table, -1 when cells are read from HBM
   double scales[16];
 };
 
@@ -73,6 +78,7 @@ struct Ctx {
   double* d_temp = nullptr; double* d_fleck = nullptr; double* d_sa = nullptr; double* d_ss = nullptr;
   double* d_radsrc = nullptr; double* d_tsurf = nullptr;   // tsurf packed (1D: 2, 2D: 2nx+2ny)
   void* d_sigcol = nullptr; void* d_sigdep = nullptr;
+  void* d_isig = nullptr; void* d_idxw = nullptr; void* d_idyw = nullptr;   // reciprocals
 
   // sourcing entries (faces, body cells, radiation-source cells)
   int64_t NE = 0;

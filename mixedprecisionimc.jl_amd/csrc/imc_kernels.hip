@@ -55,17 +55,20 @@ __global__ void k_prep(int nx, int ny, int C, double s, const double* __restrict
                        const double* __restrict__ ss, const double* __restrict__ f,
                        typename Prec<P>::T* dxs, typename Prec<P>::T* dys,
                        typename Prec<P>::W* dxw, typename Prec<P>::W* dyw,
-                       typename Prec<P>::T* sigcol, typename Prec<P>::W* sigdep) {
+                       typename Prec<P>::T* sigcol, typename Prec<P>::W* sigdep,
+                       typename Prec<P>::T* isig, typename Prec<P>::W* idxw, typename Prec<P>::W* idyw) {
 #pragma clang fp contract(off)
   using T = typename Prec<P>::T;
   using W = typename Prec<P>::W;
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < C) {
-    sigcol[i] = Tm<T>::from(sa[i] * (1.0 - f[i]) + ss[i]);   // imc_transport.jl:87
+    const double sc = sa[i] * (1.0 - f[i]) + ss[i];
+    sigcol[i] = Tm<T>::from(sc);                              // imc_transport.jl:87
+    isig[i] = Tm<T>::from(1.0 / sc);                          // :87 as a product
     sigdep[i] = (W)(sa[i] * f[i]);                            // imc_transport.jl:95
   }
-  if (i < nx) { dxs[i] = Tm<T>::from(dx[i] * s); dxw[i] = (W)dx[i]; }
-  if (i < ny) { dys[i] = Tm<T>::from(dy[i] * s); dyw[i] = (W)dy[i]; }
+  if (i < nx) { dxs[i] = Tm<T>::from(dx[i] * s); dxw[i] = (W)dx[i]; idxw[i] = (W)(1.0 / dx[i]); }
+  if (i < ny) { dys[i] = Tm<T>::from(dy[i] * s); dyw[i] = (W)dy[i]; idyw[i] = (W)(1.0 / dy[i]); }
 }
 
 hipError_t launch_prep(Ctx& c) {
@@ -76,7 +79,8 @@ hipError_t launch_prep(Ctx& c) {
   k_prep<P><<<grid, 256, 0, c.stream>>>(                                                        \
       c.nx, c.ny, c.C, s, c.d_dx, c.d_dy, c.d_sa, c.d_ss, c.d_fleck,                             \
       (Prec<P>::T*)c.d_dxs, (Prec<P>::T*)c.d_dys, (Prec<P>::W*)c.d_dxw, (Prec<P>::W*)c.d_dyw,     \
-      (Prec<P>::T*)c.d_sigcol, (Prec<P>::W*)c.d_sigdep)
+      (Prec<P>::T*)c.d_sigcol, (Prec<P>::W*)c.d_sigdep, (Prec<P>::T*)c.d_isig, (Prec<P>::W*)c.d_idxw,  \
+      (Prec<P>::W*)c.d_idyw)
   if (c.prec == 64) PREP(64);
   else if (c.prec == 32) PREP(32);
   else PREP(16);
@@ -522,15 +526,24 @@ struct SurrArgs {
 };
 
 constexpr uint32_t TAG_SURR = 3u;
+
+// The four per-cell values a 1D event reads, staged once per block in LDS when the
+// table is small (one 16-byte LDS read per event instead of four HBM/L2 gathers).
+template <int P> struct alignas(16) CellRec {
+  typename Prec<P>::T dxs, isig;
+  typename Prec<P>::W sigdep, idxw;
+};
+constexpr size_t CELL_LDS_MAX = 16 * 1024;
 constexpr size_t SURR_LDS = (size_t)BF_TOTAL * 2 + 544 + (size_t)NWAVES * sizeof(MlpScratch);
 
-template <int P, int TM, bool TWO_D, bool SURR>
+template <int P, int TM, bool TWO_D, bool SURR, bool CELL>
 __global__ __launch_bounds__(TPB) void k_transport(SoA s, Geo g, Scalars* sc, void* partial_v,
                                                    double* gbins, SurrArgs sa) {
   using T = typename Prec<P>::T;
   using W = typename Prec<P>::W;
   using Acc = typename Prec<P>::Acc;
   static_assert(!(SURR && TWO_D), "the surrogate path is 1D");
+  static_assert(!(CELL && TWO_D), "the LDS cell table is 1D");
   extern __shared__ __align__(16) unsigned char smem[];
   Acc* lds = (Acc*)smem;
   const int NB = g.NB, C = g.C, nx = g.nx, ny = g.ny;
@@ -545,15 +558,27 @@ __global__ __launch_bounds__(TPB) void k_transport(SoA s, Geo g, Scalars* sc, vo
     for (int i = threadIdx.x; i < BF_TOTAL; i += blockDim.x) s_w[i] = sa.wbf[i];
     for (int i = threadIdx.x; i < BIAS_TOTAL; i += blockDim.x) s_b[i] = sa.bias[i];
   }
-  tally_init<Acc, TM>(lds, NB);
+  const CellRec<P>* s_cell = nullptr;
+  if (CELL) {
+    CellRec<P>* w = (CellRec<P>*)(smem + g.cell_off);
+    for (int i = threadIdx.x; i < C; i += blockDim.x) {
+      CellRec<P> r;
+      r.dxs = ((const T*)g.dxs)[i]; r.isig = ((const T*)g.isig)[i];
+      r.sigdep = ((const W*)g.sigdep)[i]; r.idxw = ((const W*)g.idxw)[i];
+      w[i] = r;
+    }
+    s_cell = w;
+  }
+  tally_init<Acc, TM>(lds, NB);       // its barrier also publishes the cell table
   Tally<Acc, TM> tal{lds, gbins, NB};
 
   const T* __restrict__ dxs = (const T*)g.dxs;
   const T* __restrict__ dys = (const T*)g.dys;
-  const W* __restrict__ dxw = (const W*)g.dxw;
-  const W* __restrict__ dyw = (const W*)g.dyw;
   const T* __restrict__ sigcol = (const T*)g.sigcol;
   const W* __restrict__ sigdep = (const W*)g.sigdep;
+  const T* __restrict__ isig = (const T*)g.isig;
+  const W* __restrict__ idxw = (const W*)g.idxw;
+  const W* __restrict__ idyw = (const W*)g.idyw;
   uint32_t* __restrict__ CK = s.cellk;
   T* __restrict__ X = (T*)s.x;
   T* __restrict__ Y = (T*)s.y;
@@ -567,6 +592,11 @@ __global__ __launch_bounds__(TPB) void k_transport(SoA s, Geo g, Scalars* sc, vo
   const uint64_t lt_mask = lane == 0 ? 0ull : (~0ull >> (64 - lane));
   const T zero = (T)0, one = (T)1, two = (T)2;
   const T cT = Tm<T>::from(g.c), sT = Tm<T>::from(g.s), dtT = Tm<T>::from(g.dt);
+  const T icsT = Tm<T>::from(g.ics);   // (d/s)/c of imc_transport.jl:126 / :618 as d * (1/(c s))
+  auto dx_of = [&](int i) -> T {
+    if (CELL) return s_cell[i].dxs;
+    return dxs[i];
+  };
 
   bool has = false;
   uint32_t slot = 0, ev = 0;
@@ -654,12 +684,12 @@ __global__ __launch_bounds__(TPB) void k_transport(SoA s, Geo g, Scalars* sc, vo
       bin = k * C + ix;
       if (u24(q.r[0]) < sout[0]) {                           // stays until the budget ends
         if (s_Scut < s_ST) {                                  // 1% cut-off first: all deposited
-          val = (Acc)(E / dxw[ix]);
+          val = (Acc)(E * idxw[ix]);
           CK[slot] = DEAD;
           has = false;
         } else {                                              // census inside the cell
           const W Enew = E * Tm<W>::exp_(-(sigdep[ix] * (W)(cT * (dtT - t) * sT)));
-          val = (Acc)((E - Enew) / dxw[ix]);
+          val = (Acc)((E - Enew) * idxw[ix]);
           E = Enew;
           // position of a walker that has not left the cell: diffusion from x0 with absorbing
           // faces, spread sigma^2 = 2 S / (3 sigma_c).  Narrow spread: rejection against the
@@ -701,9 +731,9 @@ __global__ __launch_bounds__(TPB) void k_transport(SoA s, Geo g, Scalars* sc, vo
         const bool right = u24(q.r[1]) < sout[1];
         const T path = Tm<T>::from((double)(sout[2] * s_SB));
         const W Enew = E * (W)(1.0f - sout[3]);
-        val = (Acc)((E - Enew) / dxw[ix]);
+        val = (Acc)((E - Enew) * idxw[ix]);
         E = Enew;
-        t = t + (path / sT) / cT;
+        t = t + path * icsT;
         // outgoing direction cosine from a diffusive medium, p(mu) ~ mu (1 + 1.5 mu):
         // Lambertian proposal mu = sqrt(u) accepted with probability (1 + 1.5 mu)/2.5
         float m = sqrtf(fmaxf(u24(q.r[2]), 0x1p-24f));
@@ -714,12 +744,12 @@ __global__ __launch_bounds__(TPB) void k_transport(SoA s, Geo g, Scalars* sc, vo
         }
         bool dead = false;
         if (E <= Emin) {                                      // below the cut-off: deposit here
-          val += (Acc)(E / dxw[ix]);
+          val += (Acc)(E * idxw[ix]);
           CK[slot] = DEAD;
           has = false;
         } else {
           mu = Tm<T>::from(right ? (double)m : -(double)m);
-          x = right ? dxs[ix] : zero;
+          x = right ? dx_of(ix) : zero;
           if (mu > zero) {                                    // face crossing as :130-170
             if (ix == C - 1) {
               if (g.bc[1] == IMC_REFLECT) mu = -mu; else dead = true;
@@ -730,7 +760,7 @@ __global__ __launch_bounds__(TPB) void k_transport(SoA s, Geo g, Scalars* sc, vo
             if (ix == 0) {
               if (g.bc[0] == IMC_REFLECT) mu = -mu; else dead = true;
             } else {
-              ix -= 1; x = dxs[ix];
+              ix -= 1; x = dx_of(ix);
             }
           }
           if (dead) {
@@ -746,22 +776,30 @@ __global__ __launch_bounds__(TPB) void k_transport(SoA s, Geo g, Scalars* sc, vo
       ++segs;
       const u4 q = philox(slot, ev, TAG_EVENT, 0u, g.seed, g.step);
       const int cell = TWO_D ? ix + nx * iy : ix;
+      T cdx, cis;
+      W csd, cidx;
+      if (CELL) {
+        const CellRec<P> r = s_cell[ix];
+        cdx = r.dxs; cis = r.isig; csd = r.sigdep; cidx = r.idxw;
+      } else {
+        cdx = dxs[ix]; cis = isig[cell]; csd = sigdep[cell]; cidx = idxw[ix];
+      }
       T db, dbx = zero, dby = zero;
       if (!TWO_D) {
-        db = (mu > zero) ? (dxs[ix] - x) / mu : Tm<T>::abs_(x / mu);          // :77-83
+        db = (mu > zero) ? (cdx - x) / mu : Tm<T>::abs_(x / mu);              // :77-83
       } else {
-        dbx = (ox > zero) ? Tm<T>::abs_((dxs[ix] - x) / ox) : Tm<T>::abs_(x / ox);   // :538-542
+        dbx = (ox > zero) ? Tm<T>::abs_((cdx - x) / ox) : Tm<T>::abs_(x / ox);       // :538-542
         dby = (oy > zero) ? Tm<T>::abs_((dys[iy] - y) / oy) : Tm<T>::abs_(y / oy);   // :544-548
         if (Tm<T>::nan_(dbx)) db = dby;                                       // :551-557
         else if (Tm<T>::nan_(dby)) db = dbx;
         else db = jmin(dbx, dby);
       }
-      const T dc = Tm<T>::expvar(q) / sigcol[cell];                           // :87 / :561
+      const T dc = Tm<T>::expvar(q) * cis;                                    // :87 / :561
       const T dcen = cT * (dtT - t) * sT;                                     // :89 / :569
       const T d = jmin(jmin(db, dc), dcen);                                   // :92 / :571
-      const W tau = sigdep[cell] * (W)d;
+      const W tau = csd * (W)d;
       const W Enew = E * Tm<W>::exp_(-tau);                                   // :95 / :580
-      const W Evol = TWO_D ? (E / dxw[ix]) / dyw[iy] : E / dxw[ix];
+      const W Evol = TWO_D ? (E * cidx) * idyw[iy] : E * cidx;
       bin = k * C + cell;
       if (Enew <= Emin) {                                                     // :97-106 / :586-595
         val = (Acc)Evol;
@@ -770,7 +808,7 @@ __global__ __launch_bounds__(TPB) void k_transport(SoA s, Geo g, Scalars* sc, vo
       } else {
         val = (Acc)(-Evol * Tm<W>::expm1_(-tau));                              // :120 / :607
         E = Enew;
-        t = t + (d / sT) / cT;
+        t = t + d * icsT;
         bool dead = false, face = false;
         if (!TWO_D) {
           x = x + mu * d;                                                     // :124
@@ -785,7 +823,7 @@ __global__ __launch_bounds__(TPB) void k_transport(SoA s, Geo g, Scalars* sc, vo
               if (ix == 0) {
                 if (g.bc[0] == IMC_REFLECT) mu = -mu; else dead = true;
               } else {
-                ix -= 1; x = dxs[ix];
+                ix -= 1; x = dx_of(ix);
               }
             }
           }
@@ -884,51 +922,88 @@ static void pick_tally(const Ctx& c, int nbins, int* tally_mode, size_t* lds_byt
 
 static bool use_surrogate(const Ctx& c) { return c.surr_enabled && !c.is2d; }
 
-// Dispatch over (precision, tally mode, geometry, surrogate).  F(P, TM, D, S) expands to
-// a statement using the kernel k_transport<P, TM, D, S>.
-#define IMC_TRANSPORT_DISPATCH(F)                                                                    \
-  do {                                                                                               \
-    const bool surr_ = use_surrogate(c);                                                             \
-    if (c.prec == 64) {                                                                              \
-      if (c.is2d) { if (tm == TM_WAVEROW) F(64, TM_WAVEROW, true, false);                            \
-                    else if (tm == TM_BLOCKROW) F(64, TM_BLOCKROW, true, false);                     \
-                    else F(64, TM_GLOBAL, true, false); }                                            \
-      else if (surr_) { if (tm == TM_GLOBAL) F(64, TM_GLOBAL, false, true);                          \
-                        else F(64, TM_BLOCKROW, false, true); }                                      \
-      else { if (tm == TM_WAVEROW) F(64, TM_WAVEROW, false, false);                                  \
-             else if (tm == TM_BLOCKROW) F(64, TM_BLOCKROW, false, false);                           \
-             else F(64, TM_GLOBAL, false, false); }                                                  \
-    } else if (c.prec == 32) {                                                                       \
-      if (c.is2d) { if (tm == TM_WAVEROW) F(32, TM_WAVEROW, true, false);                            \
-                    else if (tm == TM_BLOCKROW) F(32, TM_BLOCKROW, true, false);                     \
-                    else F(32, TM_GLOBAL, true, false); }                                            \
-      else if (surr_) { if (tm == TM_GLOBAL) F(32, TM_GLOBAL, false, true);                          \
-                        else F(32, TM_BLOCKROW, false, true); }                                      \
-      else { if (tm == TM_WAVEROW) F(32, TM_WAVEROW, false, false);                                  \
-             else if (tm == TM_BLOCKROW) F(32, TM_BLOCKROW, false, false);                           \
-             else F(32, TM_GLOBAL, false, false); }                                                  \
-    } else {                                                                                         \
-      if (c.is2d) { if (tm == TM_WAVEROW) F(16, TM_WAVEROW, true, false);                            \
-                    else if (tm == TM_BLOCKROW) F(16, TM_BLOCKROW, true, false);                     \
-                    else F(16, TM_GLOBAL, true, false); }                                            \
-      else if (surr_) { if (tm == TM_GLOBAL) F(16, TM_GLOBAL, false, true);                          \
-                        else F(16, TM_BLOCKROW, false, true); }                                      \
-      else { if (tm == TM_WAVEROW) F(16, TM_WAVEROW, false, false);                                  \
-             else if (tm == TM_BLOCKROW) F(16, TM_BLOCKROW, false, false);                           \
-             else F(16, TM_GLOBAL, false, false); }                                                  \
-    }                                                                                                \
+// Bytes of the 1D LDS cell table, 0 when it is not used (2D, or more than CELL_LDS_MAX).
+static size_t cell_table_bytes(const Ctx& c) {
+  if (c.is2d) return 0;
+  const size_t rec = c.prec == 64 ? sizeof(CellRec<64>) : c.prec == 32 ? sizeof(CellRec<32>) : sizeof(CellRec<16>);
+  const size_t b = (size_t)c.C * rec;
+  return b <= CELL_LDS_MAX ? b : 0;
+}
+
+// Dispatch over (precision, tally mode, geometry, surrogate, LDS cell table).  F(P, TM, D, S, L)
+// expands to a statement using the kernel k_transport<P, TM, D, S, L>.
+#define IMC_TRANSPORT_DISPATCH(F)                                                        \
+  do {                                                                                   \
+    const bool surr_ = use_surrogate(c);                                                 \
+    const bool cell_ = cell_table_bytes(c) > 0;                                          \
+    if (c.prec == 64) {                                                                  \
+      if (c.is2d) { if (tm == TM_WAVEROW) F(64, TM_WAVEROW, true, false, false);         \
+                    else if (tm == TM_BLOCKROW) F(64, TM_BLOCKROW, true, false, false);  \
+                    else F(64, TM_GLOBAL, true, false, false); }                         \
+      else if (cell_) {                                                                  \
+        if (surr_) { if (tm == TM_GLOBAL) F(64, TM_GLOBAL, false, true, true);           \
+                     else F(64, TM_BLOCKROW, false, true, true); }                       \
+        else if (tm == TM_WAVEROW) F(64, TM_WAVEROW, false, false, true);                \
+        else if (tm == TM_BLOCKROW) F(64, TM_BLOCKROW, false, false, true);              \
+        else F(64, TM_GLOBAL, false, false, true);                                       \
+      }                                                                                  \
+      else {                                                                             \
+        if (surr_) { if (tm == TM_GLOBAL) F(64, TM_GLOBAL, false, true, false);          \
+                     else F(64, TM_BLOCKROW, false, true, false); }                      \
+        else if (tm == TM_WAVEROW) F(64, TM_WAVEROW, false, false, false);               \
+        else if (tm == TM_BLOCKROW) F(64, TM_BLOCKROW, false, false, false);             \
+        else F(64, TM_GLOBAL, false, false, false);                                      \
+      }                                                                                  \
+    } else if (c.prec == 32) {                                                           \
+      if (c.is2d) { if (tm == TM_WAVEROW) F(32, TM_WAVEROW, true, false, false);         \
+                    else if (tm == TM_BLOCKROW) F(32, TM_BLOCKROW, true, false, false);  \
+                    else F(32, TM_GLOBAL, true, false, false); }                         \
+      else if (cell_) {                                                                  \
+        if (surr_) { if (tm == TM_GLOBAL) F(32, TM_GLOBAL, false, true, true);           \
+                     else F(32, TM_BLOCKROW, false, true, true); }                       \
+        else if (tm == TM_WAVEROW) F(32, TM_WAVEROW, false, false, true);                \
+        else if (tm == TM_BLOCKROW) F(32, TM_BLOCKROW, false, false, true);              \
+        else F(32, TM_GLOBAL, false, false, true);                                       \
+      }                                                                                  \
+      else {                                                                             \
+        if (surr_) { if (tm == TM_GLOBAL) F(32, TM_GLOBAL, false, true, false);          \
+                     else F(32, TM_BLOCKROW, false, true, false); }                      \
+        else if (tm == TM_WAVEROW) F(32, TM_WAVEROW, false, false, false);               \
+        else if (tm == TM_BLOCKROW) F(32, TM_BLOCKROW, false, false, false);             \
+        else F(32, TM_GLOBAL, false, false, false);                                      \
+      }                                                                                  \
+    } else {                                                                             \
+      if (c.is2d) { if (tm == TM_WAVEROW) F(16, TM_WAVEROW, true, false, false);         \
+                    else if (tm == TM_BLOCKROW) F(16, TM_BLOCKROW, true, false, false);  \
+                    else F(16, TM_GLOBAL, true, false, false); }                         \
+      else if (cell_) {                                                                  \
+        if (surr_) { if (tm == TM_GLOBAL) F(16, TM_GLOBAL, false, true, true);           \
+                     else F(16, TM_BLOCKROW, false, true, true); }                       \
+        else if (tm == TM_WAVEROW) F(16, TM_WAVEROW, false, false, true);                \
+        else if (tm == TM_BLOCKROW) F(16, TM_BLOCKROW, false, false, true);              \
+        else F(16, TM_GLOBAL, false, false, true);                                       \
+      }                                                                                  \
+      else {                                                                             \
+        if (surr_) { if (tm == TM_GLOBAL) F(16, TM_GLOBAL, false, true, false);          \
+                     else F(16, TM_BLOCKROW, false, true, false); }                      \
+        else if (tm == TM_WAVEROW) F(16, TM_WAVEROW, false, false, false);               \
+        else if (tm == TM_BLOCKROW) F(16, TM_BLOCKROW, false, false, false);             \
+        else F(16, TM_GLOBAL, false, false, false);                                      \
+      }                                                                                  \
+    }                                                                                    \
   } while (0)
 
 int transport_choose(const Ctx& c, int* grid_blocks, int* tally_mode, size_t* lds_bytes) {
   pick_tally(c, c.NB, tally_mode, lds_bytes);
   if (use_surrogate(c) && *tally_mode == TM_WAVEROW) pick_tally(c, c.NB, tally_mode, lds_bytes);
   if (use_surrogate(c) && *tally_mode == TM_WAVEROW) { *tally_mode = TM_GLOBAL; *lds_bytes = 0; }
+  *lds_bytes += cell_table_bytes(c);   // LDS layout: [tally][cell table][surrogate]
   const size_t total = *lds_bytes + (use_surrogate(c) ? SURR_LDS : 0);
   const int tm = *tally_mode;
   int per_cu = 0;
   hipError_t err = hipSuccess;
-#define OCC(P, TM_, D, S) \
-  err = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_transport<P, TM_, D, S>, TPB, total)
+#define OCC(P, TM_, D, S, L) \
+  err = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_transport<P, TM_, D, S, L>, TPB, total)
   IMC_TRANSPORT_DISPATCH(OCC);
 #undef OCC
   if (err != hipSuccess || per_cu <= 0) per_cu = 1;
@@ -937,7 +1012,9 @@ int transport_choose(const Ctx& c, int* grid_blocks, int* tally_mode, size_t* ld
 }
 
 hipError_t launch_transport(Ctx& c, int grid, int tm, size_t lds) {
-  const Geo g = make_geo(c);
+  Geo g = make_geo(c);
+  const size_t cb = cell_table_bytes(c);
+  g.cell_off = cb ? (int)(lds - cb) : -1;
   SoA s = c.buf[c.cur];
   SurrArgs sa;
   sa.wbf = c.d_surr_wbf;
@@ -945,8 +1022,8 @@ hipError_t launch_transport(Ctx& c, int grid, int tm, size_t lds) {
   sa.box = c.surr_box;
   sa.tally_bytes = (int)lds;
   const size_t total = lds + (use_surrogate(c) ? SURR_LDS : 0);
-#define TR(P, TM_, D, S) \
-  k_transport<P, TM_, D, S><<<grid, TPB, total, c.stream>>>(s, g, c.d_sc, c.d_partial, c.d_dep, sa)
+#define TR(P, TM_, D, S, L) \
+  k_transport<P, TM_, D, S, L><<<grid, TPB, total, c.stream>>>(s, g, c.d_sc, c.d_partial, c.d_dep, sa)
   IMC_TRANSPORT_DISPATCH(TR);
 #undef TR
   return hipGetLastError();

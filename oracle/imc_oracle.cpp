@@ -427,16 +427,17 @@ template <class T, class W>
 void transport1d_t(const orc_problem* pb, const orc_step* st, uint32_t step, double* rec, int64_t n,
                    double* dep, double* lost_out, int64_t* segs_out, int nth) {
   const int C = pb->nx, K = pb->K;
-  std::vector<T> dxs(C), sigcol(C);
-  std::vector<W> sigdep(C), dxw(C);
+  std::vector<T> dxs(C), isig(C);
+  std::vector<W> sigdep(C), idxw(C);
   for (int i = 0; i < C; ++i) {
     const double sa = st->sigma_a[i], f = st->fleck[i], ss = st->sigma_s[i];
     dxs[i] = Tr<T>::from(st->dx[i] * pb->distancescale);
-    sigcol[i] = Tr<T>::from(sa * (1.0 - f) + ss);          // :87 effective scattering
+    isig[i] = Tr<T>::from(1.0 / (sa * (1.0 - f) + ss));    // :87 1/(effective scattering)
     sigdep[i] = (W)(sa * f);                                // :95 / :120 sigma_a*f
-    dxw[i] = (W)st->dx[i];
+    idxw[i] = (W)(1.0 / st->dx[i]);                         // :104 / :120 1/dx
   }
   const T cT = Tr<T>::from(pb->phys_c), sT = Tr<T>::from(pb->distancescale), dtT = Tr<T>::from(st->dt);
+  const T icsT = Tr<T>::from(1.0 / (pb->phys_c * pb->distancescale));   // :126 (d/s)/c as d*(1/(c s))
   const T zero = Tr<T>::zero(), one = Tr<T>::one(), two = Tr<T>::from(2.0);
   nth = nthreads_of(nth);
   std::vector<std::vector<double>> tdep(nth, std::vector<double>((size_t)C * K, 0.0));
@@ -461,19 +462,19 @@ void transport1d_t(const orc_problem* pb, const orc_step* st, uint32_t step, dou
       ++segs;                                               // :73
       const U4 q = philox((uint32_t)p, ev, TAG_EVENT, 0u, seed, step);
       const T db = (mu > zero) ? (dxs[i] - x) / mu : Tr<T>::abs(x / mu);   // :77-83
-      const T dc = Tr<T>::expvar(q) / sigcol[i];            // :87
+      const T dc = Tr<T>::expvar(q) * isig[i];              // :87
       const T dcen = cT * (dtT - t) * sT;                   // :89
       const T d = jmin(jmin(db, dc), dcen);                 // :92
       const W tau = sigdep[i] * (W)Tr<T>::d(d);
       const W Enew = E * (W)Tr<W>::exp(-tau);               // :95
       if (Enew <= Emin) {                                   // :97-106
-        D[k * C + i] += (double)(E / dxw[i]);
+        D[k * C + i] += (double)(E * idxw[i]);
         r[7] = -1.0;
         break;
       }
-      D[k * C + i] += (double)(-(E / dxw[i]) * (W)Tr<W>::expm1(-tau));   // :120
+      D[k * C + i] += (double)(-(E * idxw[i]) * (W)Tr<W>::expm1(-tau));   // :120
       x = x + mu * d;                                       // :124-126
-      t = t + (d / sT) / cT;
+      t = t + d * icsT;
       E = Enew;
       if (d == db) {                                        // :130-170
         bool dead = false;
@@ -525,16 +526,17 @@ template <class T, class W>
 void transport2d_t(const orc_problem* pb, const orc_step* st, uint32_t step, double* rec, int64_t n,
                    double* dep, double* lost_out, int64_t* segs_out, int nth) {
   const int nx = pb->nx, ny = pb->ny, C = nx * ny, K = pb->K;
-  std::vector<T> dxs(nx), dys(ny), sigcol(C);
-  std::vector<W> sigdep(C), dxw(nx), dyw(ny);
-  for (int i = 0; i < nx; ++i) { dxs[i] = Tr<T>::from(st->dx[i] * pb->distancescale); dxw[i] = (W)st->dx[i]; }
-  for (int j = 0; j < ny; ++j) { dys[j] = Tr<T>::from(st->dy[j] * pb->distancescale); dyw[j] = (W)st->dy[j]; }
+  std::vector<T> dxs(nx), dys(ny), isig(C);
+  std::vector<W> sigdep(C), idxw(nx), idyw(ny);
+  for (int i = 0; i < nx; ++i) { dxs[i] = Tr<T>::from(st->dx[i] * pb->distancescale); idxw[i] = (W)(1.0 / st->dx[i]); }
+  for (int j = 0; j < ny; ++j) { dys[j] = Tr<T>::from(st->dy[j] * pb->distancescale); idyw[j] = (W)(1.0 / st->dy[j]); }
   for (int c = 0; c < C; ++c) {
     const double sa = st->sigma_a[c], f = st->fleck[c], ss = st->sigma_s[c];
-    sigcol[c] = Tr<T>::from(sa * (1.0 - f) + ss);           // :561
+    isig[c] = Tr<T>::from(1.0 / (sa * (1.0 - f) + ss));    // :561
     sigdep[c] = (W)(sa * f);                                // :580 / :607
   }
   const T cT = Tr<T>::from(pb->phys_c), sT = Tr<T>::from(pb->distancescale), dtT = Tr<T>::from(st->dt);
+  const T icsT = Tr<T>::from(1.0 / (pb->phys_c * pb->distancescale));   // :618 (d/s)/c as d*(1/(c s))
   const T zero = Tr<T>::zero();
   nth = nthreads_of(nth);
   std::vector<std::vector<double>> tdep(nth, std::vector<double>((size_t)C * K, 0.0));
@@ -566,20 +568,20 @@ void transport2d_t(const orc_problem* pb, const orc_step* st, uint32_t step, dou
       if (Tr<T>::isnan(dbx)) db = dby;                      // :551-557
       else if (Tr<T>::isnan(dby)) db = dbx;
       else db = jmin(dbx, dby);
-      const T dc = Tr<T>::expvar(q) / sigcol[c];            // :561
+      const T dc = Tr<T>::expvar(q) * isig[c];              // :561
       const T dcen = cT * (dtT - t) * sT;                   // :569
       const T d = jmin(jmin(db, dc), dcen);                 // :571
       const W tau = sigdep[c] * (W)Tr<T>::d(d);
       const W Enew = E * (W)Tr<W>::exp(-tau);               // :580
       if (Enew <= Emin) {                                   // :586-595
-        D[k * C + c] += (double)((E / dxw[ix]) / dyw[iy]);
+        D[k * C + c] += (double)((E * idxw[ix]) * idyw[iy]);
         r[7] = -1.0;
         break;
       }
-      D[k * C + c] += (double)(-((E / dxw[ix]) / dyw[iy]) * (W)Tr<W>::expm1(-tau));   // :607
+      D[k * C + c] += (double)(-((E * idxw[ix]) * idyw[iy]) * (W)Tr<W>::expm1(-tau));   // :607
       x = x + d * ox;                                       // :615-618
       y = y + d * oy;
-      t = t + (d / sT) / cT;
+      t = t + d * icsT;
       E = Enew;
       if (d == dbx || d == dby) {                           // :621-704
         bool dead = false;
@@ -771,11 +773,14 @@ int orc_run(const orc_run_cfg* cfg, orc_run_out* out) {
     n += nb;
     int64_t segs = 0;
     orc_transport(pb, &st, stepidx, rec.data(), n, dep.data(), &lost, &segs, cfg->nthreads);
-    out->histories += n;
-    out->segments += segs;
+    const bool counted = out->steps_done >= cfg->time_from;
+    if (counted) {
+      out->histories += n;
+      out->segments += segs;
+    }
     n = orc_clean(pb, rec.data(), n);
     orc_census_tally(pb, &st, rec.data(), n, rad.data());
-    out->t_hot += std::chrono::duration<double>(clk::now() - t0).count();
+    if (counted) out->t_hot += std::chrono::duration<double>(clk::now() - t0).count();
     // Tally.tally (host part)
     if (t == 0.0)
       for (int i = 0; i < C; ++i)

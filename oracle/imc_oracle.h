@@ -83,6 +83,7 @@ typedef struct {
   int64_t n_input, n_max;
   double cellmin;
   int64_t max_steps;                         /* 0: run to t_end */
+  int64_t time_from;                         /* steps before this index are run but not counted */
 } orc_run_cfg;
 
 typedef struct {
@@ -94,7 +95,7 @@ typedef struct {
   int64_t steps_done;
   int64_t histories;
   int64_t segments;
-  double t_hot;            /* wall seconds in source+transport+clean+census tally */
+  double t_hot;            /* wall seconds in source+transport+clean+census tally, steps >= time_from */
   double t_final;
   int64_t n_alive;
 } orc_run_out;

@@ -46,7 +46,7 @@ class RunCfg(ctypes.Structure):
         ("nthreads", ctypes.c_int32),
         ("dt", ctypes.c_double), ("t_end", ctypes.c_double), ("dt0", ctypes.c_double), ("k", ctypes.c_double),
         ("dtmax", ctypes.c_double), ("n_input", ctypes.c_int64), ("n_max", ctypes.c_int64),
-        ("cellmin", ctypes.c_double), ("max_steps", ctypes.c_int64),
+        ("cellmin", ctypes.c_double), ("max_steps", ctypes.c_int64), ("time_from", ctypes.c_int64),
     ]
 
 
@@ -187,8 +187,9 @@ class OracleProblem:
 def run(*, geometry, prec, nx, ny=1, dx, dy=None, sa0, sa_pow, ss0, ss_pow, bee, radsource, tsurf, t_init,
         alpha, phys_c, phys_a, distancescale=1.0, scales=(1.0,), bc=(VACUUM,) * 4, seed=1234, linearized=False,
         marshak=False, timestepping="CONSTANT", dt=0.0, t_end=1.0, dt0=0.0, k=1.0, dtmax=0.0, n_input=1000,
-        n_max=10000, cellmin=1.0, max_steps=0, nthreads=1, quirks=0):
-    """Full run of the restated main loop; returns a dict of final fields and counters."""
+        n_max=10000, cellmin=1.0, max_steps=0, time_from=0, nthreads=1, quirks=0):
+    """Full run of the restated main loop; returns a dict of final fields and counters.
+    histories, segments and t_hot count the steps from index time_from on."""
     C = nx * (ny if geometry == 2 else 1)
     cfg = RunCfg()
     pb = cfg.pb
@@ -223,6 +224,7 @@ def run(*, geometry, prec, nx, ny=1, dx, dy=None, sa0, sa_pow, ss0, ss_pow, bee,
     cfg.nthreads = nthreads
     cfg.dt = dt; cfg.t_end = t_end; cfg.dt0 = dt0; cfg.k = k; cfg.dtmax = dtmax
     cfg.n_input = int(n_input); cfg.n_max = int(n_max); cfg.cellmin = cellmin; cfg.max_steps = max_steps
+    cfg.time_from = time_from
     out = RunOut()
     temp = np.zeros(C); rad = np.zeros(C); mat = np.zeros(C)
     cap = 100000
@@ -236,7 +238,7 @@ def run(*, geometry, prec, nx, ny=1, dx, dy=None, sa0, sa_pow, ss0, ss_pow, bee,
             "t_final": out.t_final, "n_alive": out.n_alive}
 
 
-def run_deck(inputs: dict, mesh, *, prec=None, seed=None, max_steps=0, nthreads=1, quirks=0):
+def run_deck(inputs: dict, mesh, *, prec=None, seed=None, max_steps=0, time_from=0, nthreads=1, quirks=0):
     """Runs the oracle's restated main loop on a deck parsed by the host package
     (Input.readInputs + Mesh.mesh_generation); used by tests and the CPU baseline."""
     def g(key):
@@ -269,4 +271,4 @@ def run_deck(inputs: dict, mesh, *, prec=None, seed=None, max_steps=0, nthreads=
                linearized=g("LINEARIZED") == "TRUE",
                marshak=(geometry == 1 and g("NAME").upper() == "MARSHAK WAVE"), timestepping=ts,
                t_end=float(g("ENDTIME")), n_input=int(float(g("NINPUT"))), n_max=int(float(g("NMAX"))),
-               cellmin=float(g("CELLMIN")), max_steps=max_steps, nthreads=nthreads, quirks=quirks, **kw)
+               cellmin=float(g("CELLMIN")), max_steps=max_steps, time_from=time_from, nthreads=nthreads, quirks=quirks, **kw)
