@@ -193,6 +193,47 @@ def surrogate_leg(steps=20, n_input=400000, seeds=4):
     return out
 
 
+DECK_CHANNEL = os.path.join(ROOT, "decks", "c3_channel_fp16.txt")
+
+
+def precision_study(steps=50, seeds=4, n_input=None):
+    """2D channel workload (280x80 graded mesh): FP16 / FP32 / FP64 tracking against an FP64
+    reference with 4x the particles.  eps = relative RMS over cells of the material
+    temperature after `steps` steps (mean over seeds), with the reference's own standard
+    error reported beside it."""
+    import numpy as np
+    import MixedPrecisionIMC as M
+    from MixedPrecisionIMC import Input
+
+    def run(prec, scale, seed):
+        inp = Input.readInputs(DECK_CHANNEL)
+        n_in = int(float(Input.get_scalar(inp, "NINPUT"))) if n_input is None else n_input
+        inp["NINPUT"] = str(int(n_in * scale))
+        inp["NMAX"] = str(int(float(Input.get_scalar(inp, "NMAX")) * scale))
+        mesh, sv, parts = M.setup(inp, precision=prec, seed=seed)
+        for _ in range(steps):
+            M.step(inp, mesh, sv, parts, "CONSTANT")
+        tm = parts.timing()
+        parts.close()
+        return mesh.temp.ravel(order="F").copy(), tm, max(abs(e) for e in sv.energy_errors)
+
+    ref_runs = [run(64, 4, 7000 + s) for s in range(seeds)]
+    ref = np.mean([r[0] for r in ref_runs], axis=0)
+    ref_se = np.std([r[0] for r in ref_runs], axis=0, ddof=1) / np.sqrt(seeds)
+    out = {"workload": f"2D channel (decks/c3_channel_fp16.txt), {steps} steps, {seeds} seeds per mode; "
+                       f"reference: FP64 with 4x NINPUT/NMAX, {seeds} seeds",
+           "reference_rel_se_rms": float(np.sqrt(np.mean((ref_se / ref) ** 2)))}
+    for prec in (16, 32, 64):
+        rs = [run(prec, 1, 100 * prec + s) for s in range(seeds)]
+        eps = [float(np.sqrt(np.mean(((r[0] - ref) / ref) ** 2))) for r in rs]
+        hot = sum((r[1]["ms_source"] + r[1]["ms_transport"] + r[1]["ms_clean"] + r[1]["ms_tally"]) / 1e3 for r in rs)
+        out[f"fp{prec}"] = {"eps_rms": float(np.mean(eps)), "eps_per_seed": eps,
+                            "histories_per_s": sum(r[1]["histories"] for r in rs) / hot,
+                            "max_energy_error": max(r[2] for r in rs)}
+        print(f"[bench] precision study fp{prec}: eps {np.mean(eps):.4g}", file=sys.stderr, flush=True)
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -202,6 +243,9 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=12)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-surrogate", action="store_true")
+    ap.add_argument("--blocks-per-cu", type=int, default=0, help="transport grid bound (0: automatic)")
+    ap.add_argument("--precision-study", action="store_true",
+                    help="also run the FP16/FP32/FP64 tally-error study on the 2D channel deck")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the engine runs on a "
@@ -212,7 +256,7 @@ def main():
     from MixedPrecisionIMC import Input
 
     inputs = Input.readInputs(args.deck)
-    mesh, simvars, particles = M.setup(inputs)
+    mesh, simvars, particles = M.setup(inputs, tuning={"blocks_per_cu": args.blocks_per_cu})
     timestepping = Input.get_scalar(inputs, "TIMESTEPPING").upper()
     for _ in range(args.warmup):
         M.step(inputs, mesh, simvars, particles, timestepping)
@@ -280,6 +324,8 @@ def main():
         line["surrogate"] = surrogate_leg()
         print(f"[bench] surrogate leg: FOM speedup {line['surrogate']['fom_speedup']:.2f}", file=sys.stderr,
               flush=True)
+    if args.precision_study:
+        line["precision_study"] = precision_study()
     print(json.dumps(line))
 
 

@@ -50,6 +50,9 @@ typedef struct {
   double energyscales[16];   /* descending (imc_mesh.jl:130) */
   double phys_c, phys_a, distancescale;  /* constants.jl:9-12, imc_mesh.jl:131 */
   int64_t capacity;          /* initial particle capacity; grows on demand */
+  /* Tuning, 0 = automatic.  reserved[0]: transport tally mode (1 per-wave LDS rows,
+     2 per-block LDS row, 3 global FP64 atomics).  reserved[1]: transport blocks per CU
+     (upper bound on the occupancy-derived persistent grid; 0 means 5).  Others must be 0. */
   int32_t reserved[8];
 } imc_config;
 

@@ -77,7 +77,7 @@ def _bc(name: str) -> int:
 
 
 def setup(inputs: dict, *, precision: Optional[int] = None, seed: Optional[int] = None,
-          capacity: Optional[int] = None, quirks: int = 0):
+          capacity: Optional[int] = None, quirks: int = 0, tuning: Optional[dict] = None):
     """Builds (mesh, simvars, particles) from a parsed deck (MixedPrecisionIMC.jl:59-134)."""
     Constants.set_constants(inputs)
     mesh = Mesh.mesh_generation(inputs)
@@ -112,7 +112,8 @@ def setup(inputs: dict, *, precision: Optional[int] = None, seed: Optional[int] 
     cap = capacity if capacity is not None else int(1.25 * (n_max + n_input) + 4 * C * (cellmin + 1) + 4096)
     particles = Engine(geometry=1 if geometry == "1D" else 2, precision=prec, nx=nx, ny=ny,
                        energyscales=mesh.energyscales, bc=BC, seed=sd, phys_c=Constants.phys_c,
-                       phys_a=Constants.phys_a, distancescale=mesh.distancescale, capacity=cap, quirks=quirks)
+                       phys_a=Constants.phys_a, distancescale=mesh.distancescale, capacity=cap, quirks=quirks,
+                       **(tuning or {}))
     particles.set_geometry(mesh.dx, mesh.dy if geometry == "2D" else None)
     return mesh, simvars, particles
 

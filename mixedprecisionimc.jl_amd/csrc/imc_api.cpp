@@ -232,7 +232,7 @@ int imc_destroy(imc_ctx* h) {
   void* ptrs[] = {c.d_dx, c.d_dy, c.d_dxs, c.d_dys, c.d_dxw, c.d_dyw, c.d_temp, c.d_fleck, c.d_sa, c.d_ss,
                   c.d_radsrc, c.d_tsurf, c.d_sigcol, c.d_sigdep, c.d_isig, c.d_idxw, c.d_idyw, c.d_ent_e, c.d_ent_esc, c.d_ent_term,
                   c.d_ent_k, c.d_ent_cnt, c.d_ent_off, c.d_red, c.d_emitted, c.d_dep, c.d_rad, c.d_sc,
-                  c.d_partial, c.d_tile, c.d_surr_feat, c.d_surr_targ, c.d_surr_params, c.d_surr_grad,
+                  c.d_partial, c.d_tile, c.d_bhist, c.d_surr_feat, c.d_surr_targ, c.d_surr_params, c.d_surr_grad,
                   c.d_surr_m, c.d_surr_v, c.d_surr_wbf, c.d_surr_bias, c.d_surr_loss, c.d_surr_events};
   for (void* p : ptrs) (void)hipFree(p);
   hipEvent_t evs[] = {c.ev0, c.ev1, c.evk0, c.evk1};
@@ -322,6 +322,15 @@ int imc_source(imc_ctx* h, int64_t n_input, int64_t n_max, double cellmin, doubl
   const int64_t need = c.h_sc.n_census + c.h_sc.n_births;
   rc = grow(c, need, c.h_sc.n_census);
   if (rc) return rc;
+  const int64_t hl = birth_hist_len(c.h_sc.n_births);
+  if (hl > c.bhist_len) {                          // per-tile bucket counts of the birth order
+    (void)hipFree(c.d_bhist);
+    c.d_bhist = nullptr;
+    c.bhist_len = 0;
+    if (dalloc(&c.d_bhist, sizeof(int64_t) * (size_t)(hl + hl / 4)) != hipSuccess)
+      return fail(c, IMC_ENOMEM, "birth-order scratch allocation failed");
+    c.bhist_len = hl + hl / 4;
+  }
   HIPCHK(launch_births(c, c.h_sc.n_births), "birth kernel");
   if (emittedenergy)
     HIPCHK(hipMemcpyAsync(emittedenergy, c.d_emitted, 8 * (size_t)c.C * c.K, hipMemcpyDeviceToHost, c.stream), "emitted D2H");

@@ -104,6 +104,8 @@ struct Ctx {
   double* d_dep = nullptr;         // NB = C*K + 1 (last bin: lost energy)
   double* d_rad = nullptr;         // C
   int64_t* d_tile = nullptr;       // clean: per-tile counts/offsets
+  int64_t* d_bhist = nullptr;      // births: per-tile start-time bucket counts/offsets
+  int64_t bhist_len = 0;
   int64_t tile_len = 0;
 
   // learned thick-cell surrogate
@@ -136,6 +138,7 @@ hipError_t launch_source_entries(Ctx& c);
 hipError_t launch_tree_sum(Ctx& c, const double* in, int64_t n, double* out_scalar);
 hipError_t launch_counts_and_scan(Ctx& c, int64_t n_input, int64_t n_max, double cellmin);
 hipError_t launch_births(Ctx& c, int64_t n_births);
+int64_t birth_hist_len(int64_t n_births);
 hipError_t launch_transport(Ctx& c, int grid_blocks, int tally_mode, size_t lds_bytes);
 hipError_t launch_clean(Ctx& c, int64_t n_total);
 hipError_t launch_reduce_partials(Ctx& c, int nblocks, int tally_mode, int nbins, double* out);

@@ -326,86 +326,202 @@ struct BirthArgs {
   SoA out;
 };
 
-template <int P, bool TWO_D>
-__global__ __launch_bounds__(256) void k_births(BirthArgs A) {
-#pragma clang fp contract(off)
-  using T = typename Prec<P>::T;
-  using W = typename Prec<P>::W;
-  using Cc = typename Prec<P>::Cc;
-  const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (b >= A.n_births) return;
-  // entry: last e with off[e] <= b
+// One birth's state (everything but the slot it goes to).
+template <int P> struct Birth {
+  typename Prec<P>::T x, y, mu, t;
+  uint32_t cell;
+};
+
+// Entry of birth b: last e with off[e] <= b.
+__device__ __forceinline__ int64_t birth_entry(const BirthArgs& A, int64_t b) {
   int64_t lo = 0, hi = A.NE - 1;
   while (lo < hi) {
     const int64_t mid = (lo + hi + 1) >> 1;
     if (A.ent_off[mid] <= b) lo = mid; else hi = mid - 1;
   }
-  const int64_t e = lo;
+  return lo;
+}
+
+template <int P, bool TWO_D>
+__device__ __forceinline__ Birth<P> birth_state(const BirthArgs& A, int64_t b, int64_t e) {
+#pragma clang fp contract(off)
+  using T = typename Prec<P>::T;
+  using W = typename Prec<P>::W;
+  using Cc = typename Prec<P>::Cc;
   const uint64_t ub = (uint64_t)b;
-  const W nrg = (W)A.ent_nrg[e];
-  const int k = A.ent_k[e];
   const Cc dtc = (Cc)A.dt, sc = (Cc)A.s;
   const double sW = (double)(W)A.s;
-  const int64_t slot = A.sc->n_census + b;
-  T* X = (T*)A.out.x; T* MU = (T*)A.out.mu; T* TT = (T*)A.out.t;
-  W* EE = (W*)A.out.E; W* E0 = (W*)A.out.E0;
-  uint32_t cell;
+  Birth<P> o;
+  o.y = Tm<T>::from(0.0);
   if (!TWO_D) {
     const int C = A.C;
-    T x, mu, t;
     if (e < 2) {                                             // surface births :161-189
       const bool left = (e == 0);
-      cell = left ? 0u : (uint32_t)(C - 1);
-      const double dxe = (double)(W)A.dx[cell];
-      x = Tm<T>::from((left ? 0.01 : 0.99) * dxe * sW);
+      o.cell = left ? 0u : (uint32_t)(C - 1);
+      const double dxe = (double)(W)A.dx[o.cell];
+      o.x = Tm<T>::from((left ? 0.01 : 0.99) * dxe * sW);
       Cc m = BirthU<Cc>::sqrt_(BirthU<Cc>::u(ub, 1, A.seed, A.step));
       for (int at = 0; m == (Cc)0; ++at) m = BirthU<Cc>::sqrt_(BirthU<Cc>::u(ub, 8 + at, A.seed, A.step));
-      mu = Tm<T>::from(left ? (double)m : -(double)m);
-      t = Tm<T>::from((double)(dtc * BirthU<Cc>::u(ub, 2, A.seed, A.step)));
+      o.mu = Tm<T>::from(left ? (double)m : -(double)m);
+      o.t = Tm<T>::from((double)(dtc * BirthU<Cc>::u(ub, 2, A.seed, A.step)));
     } else {                                                 // body :193-214, source :217-236
-      cell = (uint32_t)((e - 2) % C);
-      x = Tm<T>::from((double)((Cc)A.dx[cell] * BirthU<Cc>::u(ub, 0, A.seed, A.step) * sc));
+      o.cell = (uint32_t)((e - 2) % C);
+      o.x = Tm<T>::from((double)((Cc)A.dx[o.cell] * BirthU<Cc>::u(ub, 0, A.seed, A.step) * sc));
       Cc m = (Cc)1 - (Cc)2 * BirthU<Cc>::u(ub, 1, A.seed, A.step);
       for (int at = 0; m == (Cc)0; ++at) m = (Cc)1 - (Cc)2 * BirthU<Cc>::u(ub, 8 + at, A.seed, A.step);
-      mu = Tm<T>::from((double)m);
-      t = Tm<T>::from((double)(dtc * BirthU<Cc>::u(ub, 2, A.seed, A.step)));
+      o.mu = Tm<T>::from((double)m);
+      o.t = Tm<T>::from((double)(dtc * BirthU<Cc>::u(ub, 2, A.seed, A.step)));
     }
-    X[slot] = x; MU[slot] = mu; TT[slot] = t;
   } else {
     const int nx = A.nx, ny = A.ny;
     const int ei = (int)e;
     int ix, iy;
-    T x, y, th, t;
     if (ei < 2 * nx) {                                       // bottom/top :265-293
       const bool bot = ei < nx;
       ix = ei % nx; iy = bot ? 0 : ny - 1;
-      t = Tm<T>::from((double)(dtc * BirthU<Cc>::u(ub, 0, A.seed, A.step)));
-      x = Tm<T>::from((double)((Cc)A.dx[ix] * BirthU<Cc>::u(ub, 1, A.seed, A.step) * sc));
-      y = Tm<T>::from((bot ? 0.001 : 0.999) * (double)(W)A.dy[iy] * sW);
-      th = Tm<T>::from((bot ? PI_D : -PI_D) * (double)BirthU<Cc>::u(ub, 2, A.seed, A.step));
+      o.t = Tm<T>::from((double)(dtc * BirthU<Cc>::u(ub, 0, A.seed, A.step)));
+      o.x = Tm<T>::from((double)((Cc)A.dx[ix] * BirthU<Cc>::u(ub, 1, A.seed, A.step) * sc));
+      o.y = Tm<T>::from((bot ? 0.001 : 0.999) * (double)(W)A.dy[iy] * sW);
+      o.mu = Tm<T>::from((bot ? PI_D : -PI_D) * (double)BirthU<Cc>::u(ub, 2, A.seed, A.step));
     } else if (ei < 2 * nx + 2 * ny) {                       // left/right :295-323
       const bool lf = ei < 2 * nx + ny;
       iy = (ei - 2 * nx) % ny; ix = lf ? 0 : nx - 1;
-      t = Tm<T>::from((double)(dtc * BirthU<Cc>::u(ub, 0, A.seed, A.step)));
-      x = Tm<T>::from((lf ? 0.001 : 0.999) * (double)(W)A.dx[ix] * sW);
+      o.t = Tm<T>::from((double)(dtc * BirthU<Cc>::u(ub, 0, A.seed, A.step)));
+      o.x = Tm<T>::from((lf ? 0.001 : 0.999) * (double)(W)A.dx[ix] * sW);
       const double wy = (A.quirks & IMC_QUIRK_Q2_LR_FACE_DX) ? A.dx[iy] : A.dy[iy];
-      y = Tm<T>::from((double)((Cc)wy * BirthU<Cc>::u(ub, 1, A.seed, A.step) * sc));
+      o.y = Tm<T>::from((double)((Cc)wy * BirthU<Cc>::u(ub, 1, A.seed, A.step) * sc));
       const double u2 = (double)BirthU<Cc>::u(ub, 2, A.seed, A.step);
-      th = Tm<T>::from(PI_D * (lf ? (0.5 - u2) : (0.5 + u2)));
+      o.mu = Tm<T>::from(PI_D * (lf ? (0.5 - u2) : (0.5 + u2)));
     } else {                                                 // body / source :326-366
       const int cellc = (int)((e - (2 * nx + 2 * ny)) % A.C);
       ix = cellc % nx; iy = cellc / nx;
-      x = Tm<T>::from((double)((Cc)A.dx[ix] * BirthU<Cc>::u(ub, 0, A.seed, A.step) * sc));
-      y = Tm<T>::from((double)((Cc)A.dy[iy] * BirthU<Cc>::u(ub, 1, A.seed, A.step) * sc));
-      th = Tm<T>::from(TWO_PI_D * (double)BirthU<Cc>::u(ub, 2, A.seed, A.step));
-      t = Tm<T>::from((double)(dtc * BirthU<Cc>::u(ub, 3, A.seed, A.step)));
+      o.x = Tm<T>::from((double)((Cc)A.dx[ix] * BirthU<Cc>::u(ub, 0, A.seed, A.step) * sc));
+      o.y = Tm<T>::from((double)((Cc)A.dy[iy] * BirthU<Cc>::u(ub, 1, A.seed, A.step) * sc));
+      o.mu = Tm<T>::from(TWO_PI_D * (double)BirthU<Cc>::u(ub, 2, A.seed, A.step));
+      o.t = Tm<T>::from((double)(dtc * BirthU<Cc>::u(ub, 3, A.seed, A.step)));
     }
-    cell = (uint32_t)(ix + nx * iy);
-    X[slot] = x; ((T*)A.out.y)[slot] = y; MU[slot] = th; TT[slot] = t;
+    o.cell = (uint32_t)(ix + nx * iy);
   }
-  A.out.cellk[slot] = cell | ((uint32_t)k << KSHIFT);
-  EE[slot] = nrg;
-  E0[slot] = nrg;
+  return o;
+}
+
+// Birth order.  The births of a step are stored in a stable order by start-time bucket
+// floor(BK t/dt): histories with the most time left (the longest on average) enter the
+// transport work queue first, which shortens the tail of the persistent launch.  The
+// order changes which slot (and so which event stream) a birth gets, not its state.
+constexpr int BK = 16;
+constexpr int BIRTH_ITEMS = 8;
+constexpr int BIRTH_TILE = 256 * BIRTH_ITEMS;
+
+__device__ __forceinline__ int t_bucket(double t, double dt) {
+  int q = (int)(t / dt * (double)BK);
+  return q < 0 ? 0 : (q > BK - 1 ? BK - 1 : q);
+}
+
+// Per-tile bucket counts, bucket-major: hist[bucket * ntiles + tile].
+template <int P, bool TWO_D>
+__global__ __launch_bounds__(256) void k_birth_hist(BirthArgs A, int64_t ntiles, int64_t* __restrict__ hist) {
+  __shared__ int cnt[BK];
+  if (threadIdx.x < BK) cnt[threadIdx.x] = 0;
+  __syncthreads();
+  const int64_t tile = blockIdx.x;
+#pragma unroll 1
+  for (int j = 0; j < BIRTH_ITEMS; ++j) {
+    const int64_t b = tile * BIRTH_TILE + (int64_t)j * 256 + threadIdx.x;
+    if (b < A.n_births) {
+      const Birth<P> o = birth_state<P, TWO_D>(A, b, birth_entry(A, b));
+      atomicAdd(&cnt[t_bucket(Tm<typename Prec<P>::T>::d(o.t), A.dt)], 1);
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x < BK) hist[(int64_t)threadIdx.x * ntiles + tile] = cnt[threadIdx.x];
+}
+
+// Single-block exclusive scan of n int64 values in place.
+__global__ __launch_bounds__(1024) void k_exscan_i64(int64_t* __restrict__ a, int64_t n) {
+  __shared__ int64_t wtot[16];
+  __shared__ int64_t running;
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  if (tid == 0) running = 0;
+  __syncthreads();
+  for (int64_t base = 0; base < n; base += 1024) {
+    const int64_t i = base + tid;
+    const int64_t v = i < n ? a[i] : 0;
+    int64_t incl = v;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+      const int64_t o = __shfl_up(incl, d, 64);
+      if (lane >= d) incl += o;
+    }
+    if (lane == 63) wtot[w] = incl;
+    __syncthreads();
+    int64_t wpre = 0;
+    for (int j = 0; j < w; ++j) wpre += wtot[j];
+    const int64_t r0 = running;
+    if (i < n) a[i] = r0 + wpre + incl - v;
+    __syncthreads();
+    if (tid == 1023) running = r0 + wpre + incl;
+    __syncthreads();
+  }
+}
+
+// One block per tile of BIRTH_TILE births (imc_sourcing.jl:161-236 in 1D, :265-366 in 2D).
+// Birth b's uniforms: Philox(b_lo, b_hi, TAG_BIRTH, group) keyed by (seed, step); draw j
+// of a birth is word j of the groups (53-bit pairs for FP64, 24-bit words otherwise);
+// direction redraws use draws 8, 9, ...  Its slot is n_census + its stable rank by
+// start-time bucket (off = scanned k_birth_hist counts).
+template <int P, bool TWO_D>
+__global__ __launch_bounds__(256) void k_births(BirthArgs A, int64_t ntiles, const int64_t* __restrict__ off) {
+  using T = typename Prec<P>::T;
+  using W = typename Prec<P>::W;
+  __shared__ int run[BK];
+  __shared__ int wc[4][BK];
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const uint64_t lt_mask = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+  if (tid < BK) run[tid] = 0;
+  const int64_t tile = blockIdx.x;
+  const int64_t n_census = A.sc->n_census;
+  T* X = (T*)A.out.x; T* MU = (T*)A.out.mu; T* TT = (T*)A.out.t;
+  W* EE = (W*)A.out.E; W* E0 = (W*)A.out.E0;
+#pragma unroll 1
+  for (int j = 0; j < BIRTH_ITEMS; ++j) {
+    const int64_t b = tile * BIRTH_TILE + (int64_t)j * 256 + tid;
+    const bool valid = b < A.n_births;
+    Birth<P> o;
+    int64_t e = 0;
+    int bk = 0;
+    if (valid) {
+      e = birth_entry(A, b);
+      o = birth_state<P, TWO_D>(A, b, e);
+      bk = t_bucket(Tm<T>::d(o.t), A.dt);
+    }
+    // lanes of this wave with the same bucket, and this lane's rank among them
+    uint64_t same = __ballot(valid);
+#pragma unroll
+    for (int bit = 0; bit < 4; ++bit) {
+      const uint64_t on = __ballot(valid && ((bk >> bit) & 1));
+      same &= ((bk >> bit) & 1) ? on : ~on;
+    }
+    const int rank = __popcll(same & lt_mask);
+    if (tid < 4 * BK) wc[tid / BK][tid % BK] = 0;
+    __syncthreads();
+    if (valid && rank == 0) wc[w][bk] = __popcll(same);
+    __syncthreads();
+    if (valid) {
+      int pre = run[bk];
+      for (int v = 0; v < w; ++v) pre += wc[v][bk];
+      const int64_t slot = n_census + off[(int64_t)bk * ntiles + tile] + pre + rank;
+      const W nrg = (W)A.ent_nrg[e];
+      X[slot] = o.x; MU[slot] = o.mu; TT[slot] = o.t;
+      if (TWO_D) ((T*)A.out.y)[slot] = o.y;
+      A.out.cellk[slot] = o.cell | ((uint32_t)A.ent_k[e] << KSHIFT);
+      EE[slot] = nrg;
+      E0[slot] = nrg;
+    }
+    __syncthreads();
+    if (tid < BK) run[tid] += wc[0][tid] + wc[1][tid] + wc[2][tid] + wc[3][tid];
+    __syncthreads();
+  }
 }
 
 hipError_t launch_births(Ctx& c, int64_t n_births) {
@@ -418,13 +534,24 @@ hipError_t launch_births(Ctx& c, int64_t n_births) {
   A.dx = c.d_dx; A.dy = c.d_dy;
   A.ent_nrg = c.d_ent_e; A.ent_esc = c.d_ent_esc; A.ent_k = c.d_ent_k; A.ent_off = c.d_ent_off;
   A.sc = c.d_sc; A.out = c.buf[c.cur];
-  const unsigned grid = (unsigned)((n_births + 255) / 256);
-#define BIR(P, D) k_births<P, D><<<grid, 256, 0, c.stream>>>(A)
+  const int64_t ntiles = (n_births + BIRTH_TILE - 1) / BIRTH_TILE;
+  if ((int64_t)BK * ntiles > c.bhist_len) return hipErrorInvalidValue;   // sized by the caller
+  const unsigned grid = (unsigned)ntiles;
+#define BIR(P, D)                                                                               \
+  do {                                                                                          \
+    k_birth_hist<P, D><<<grid, 256, 0, c.stream>>>(A, ntiles, c.d_bhist);                       \
+    k_exscan_i64<<<1, 1024, 0, c.stream>>>(c.d_bhist, (int64_t)BK * ntiles);                    \
+    k_births<P, D><<<grid, 256, 0, c.stream>>>(A, ntiles, c.d_bhist);                           \
+  } while (0)
   if (c.prec == 64) { if (c.is2d) BIR(64, true); else BIR(64, false); }
   else if (c.prec == 32) { if (c.is2d) BIR(32, true); else BIR(32, false); }
   else { if (c.is2d) BIR(16, true); else BIR(16, false); }
 #undef BIR
   return hipGetLastError();
+}
+
+int64_t birth_hist_len(int64_t n_births) {
+  return (int64_t)BK * ((n_births + BIRTH_TILE - 1) / BIRTH_TILE);
 }
 
 // ---------------------------------------------------------------------------------
@@ -786,10 +913,12 @@ __global__ __launch_bounds__(TPB) void k_transport(SoA s, Geo g, Scalars* sc, vo
       }
       T db, dbx = zero, dby = zero;
       if (!TWO_D) {
-        db = (mu > zero) ? (cdx - x) / mu : Tm<T>::abs_(x / mu);              // :77-83
+        // :77-83 with one division for both signs of mu (same values as the two branches)
+        const T qb = ((mu > zero) ? (cdx - x) : x) / mu;
+        db = (mu > zero) ? qb : Tm<T>::abs_(qb);
       } else {
-        dbx = (ox > zero) ? Tm<T>::abs_((cdx - x) / ox) : Tm<T>::abs_(x / ox);       // :538-542
-        dby = (oy > zero) ? Tm<T>::abs_((dys[iy] - y) / oy) : Tm<T>::abs_(y / oy);   // :544-548
+        dbx = Tm<T>::abs_(((ox > zero) ? (cdx - x) : x) / ox);                    // :538-542
+        dby = Tm<T>::abs_(((oy > zero) ? (dys[iy] - y) : y) / oy);                // :544-548
         if (Tm<T>::nan_(dbx)) db = dby;                                       // :551-557
         else if (Tm<T>::nan_(dby)) db = dbx;
         else db = jmin(dbx, dby);
@@ -1007,6 +1136,9 @@ int transport_choose(const Ctx& c, int* grid_blocks, int* tally_mode, size_t* ld
   IMC_TRANSPORT_DISPATCH(OCC);
 #undef OCC
   if (err != hipSuccess || per_cu <= 0) per_cu = 1;
+  // 5 blocks (20 waves) per CU saturate the SIMDs on gfx950 while leaving each lane more
+  // histories, which shortens the launch's tail; reserved[1] overrides the bound.
+  per_cu = std::min(per_cu, c.cfg.reserved[1] > 0 ? (int)c.cfg.reserved[1] : 5);
   *grid_blocks = per_cu * c.num_cu;
   return 0;
 }

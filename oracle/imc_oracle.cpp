@@ -678,15 +678,42 @@ int orc_sorter(int prec, const double* vals, int n, int promote64, const double*
   *product = (double)p; return k;
 }
 
+// Birth order of the engine: a step's births are stored in a stable order by start-time
+// bucket min(15, floor(16 t/dt)), so histories with the most time left are transported
+// first.  The reference appends births in entry order (imc_sourcing.jl:173,276); the order
+// only decides which slot, and so which event stream, a birth gets.
+static void order_births(const orc_problem* pb, const orc_step* st, double* rec, int64_t n_census, int64_t nb) {
+  if (nb <= 1) return;
+  const int RW = pb->geometry == 2 ? 10 : 9, TI = pb->geometry == 2 ? 0 : 1;
+  double* b0 = rec + (size_t)RW * n_census;
+  std::vector<int64_t> cnt(17, 0);
+  std::vector<int> key((size_t)nb);
+  for (int64_t i = 0; i < nb; ++i) {
+    int q = (int)(b0[(size_t)RW * i + TI] / st->dt * 16.0);
+    q = q < 0 ? 0 : (q > 15 ? 15 : q);
+    key[(size_t)i] = q;
+    ++cnt[(size_t)q + 1];
+  }
+  for (int q = 0; q < 16; ++q) cnt[(size_t)q + 1] += cnt[(size_t)q];
+  std::vector<double> out((size_t)RW * nb);
+  for (int64_t i = 0; i < nb; ++i) {
+    const int64_t d = cnt[(size_t)key[(size_t)i]]++;
+    std::memcpy(&out[(size_t)RW * d], b0 + (size_t)RW * i, sizeof(double) * RW);
+  }
+  std::memcpy(b0, out.data(), sizeof(double) * out.size());
+}
+
 int64_t orc_source(const orc_problem* pb, const orc_step* st, uint32_t step, int64_t n_census,
                    int64_t n_input, int64_t n_max, double cellmin, double* rec, int64_t cap,
                    double* emitted, double* etot) {
+  int64_t nb = -2;
   switch (pb->prec) {
-    case 64: return source_t<double, double, double>(pb, st, step, n_census, n_input, n_max, cellmin, rec, cap, emitted, etot);
-    case 32: return source_t<float, float, float>(pb, st, step, n_census, n_input, n_max, cellmin, rec, cap, emitted, etot);
-    case 16: return source_t<h16, float, float>(pb, st, step, n_census, n_input, n_max, cellmin, rec, cap, emitted, etot);
+    case 64: nb = source_t<double, double, double>(pb, st, step, n_census, n_input, n_max, cellmin, rec, cap, emitted, etot); break;
+    case 32: nb = source_t<float, float, float>(pb, st, step, n_census, n_input, n_max, cellmin, rec, cap, emitted, etot); break;
+    case 16: nb = source_t<h16, float, float>(pb, st, step, n_census, n_input, n_max, cellmin, rec, cap, emitted, etot); break;
   }
-  return -2;
+  if (nb > 0) order_births(pb, st, rec, n_census, nb);
+  return nb;
 }
 
 int orc_transport(const orc_problem* pb, const orc_step* st, uint32_t step, double* rec, int64_t n,

@@ -103,23 +103,25 @@ def test_source_energy_and_counts_closed_form():
     assert etot == pytest.approx(e_body.sum() + e_src.sum() + e_face.sum(), rel=1e-13)
     np.testing.assert_allclose(emitted, f * sa * a * c * T ** 4 * dt, rtol=1e-14)
     cells = rec[:, 2].astype(int) - 1
-    origin = rec[:, 0]
-    # body births: origin == cell and x in the interior (faces sit at 0.01 dx / 0.99 dx)
-    is_src = np.zeros(len(rec), bool)
-    # births are appended as [left face, right face, body cells, source cells]
     n_face_l = int(np.rint(e_face[0] * n_input / etot))
     n_face_r = int(np.rint(e_face[1] * n_input / etot))
-    body = rec[n_face_l + n_face_r:]
     n_body = np.maximum(np.rint(e_body * n_input / etot), cellmin).astype(int)
     n_src = np.where(e_src > 0, np.maximum(np.rint(e_src * n_input / etot), cellmin), 0).astype(int)
-    assert nb == n_face_l + n_face_r + n_body.sum() + n_src.sum()
-    bc = body[: n_body.sum(), 2].astype(int) - 1
-    np.testing.assert_array_equal(np.bincount(bc, minlength=C), n_body)
-    # energy per cell: sum of birth weights equals the cell's emission energy
-    np.testing.assert_allclose(np.bincount(bc, weights=body[: n_body.sum(), 6], minlength=C), e_body, rtol=1e-12)
-    sc = body[n_body.sum():, 2].astype(int) - 1
-    np.testing.assert_allclose(np.bincount(sc, weights=body[n_body.sum():, 6], minlength=C), e_src, rtol=1e-12)
-    assert not is_src.any() and origin.shape[0] == nb and cells.min() >= 0
+    assert nb == n_face_l + n_face_r + n_body.sum() + n_src.sum() == len(rec)
+    # face births sit at 0.01 dx (left) / 0.99 dx (right) of the end cells (imc_sourcing.jl:167,182)
+    s_ = case.get("s", 1.0)
+    fl = (cells == 0) & (rec[:, 3] == 0.01 * dx[0] * s_)
+    fr = (cells == C - 1) & (rec[:, 3] == 0.99 * dx[-1] * s_)
+    assert fl.sum() == n_face_l and fr.sum() == n_face_r
+    np.testing.assert_allclose([rec[fl, 6].sum(), rec[fr, 6].sum()], e_face, rtol=1e-12)
+    vol = ~(fl | fr)
+    # volume births: count and energy per cell are the body plus radiation-source closed forms
+    np.testing.assert_array_equal(np.bincount(cells[vol], minlength=C), n_body + n_src)
+    np.testing.assert_allclose(np.bincount(cells[vol], weights=rec[vol, 6], minlength=C), e_body + e_src,
+                               rtol=1e-12)
+    # births are stored in a stable order by start-time bucket floor(16 t/dt)
+    bucket = np.clip(np.floor(rec[:, 1] / dt * 16.0), 0, 15)
+    assert np.all(np.diff(bucket) >= 0)
 
 
 def test_source_sampling_distributions():
