@@ -641,9 +641,8 @@ hipError_t launch_reduce_partials(Ctx& c, int nblocks, int tally_mode, int nbins
 // or death by the 1% cut-off), otherwise it leaves through the right face with
 // P(right | exit), after the predicted mean path, with the predicted expected deposit
 // fraction, in an outward direction drawn from mu (1 + 1.5 mu).  Only particles at least
-// depth_min mean free paths from both faces use it; plain events resolve the layer next
-// to the faces.  Deposits are E_in - E_out, so every
-// history conserves energy exactly.
+// depth_min mean free paths from both faces use it; plain events resolve the layer next to
+// the faces.  Deposits are E_in - E_out, so every history conserves energy exactly.
 // ---------------------------------------------------------------------------------
 struct SurrArgs {
   const uint16_t* wbf;
@@ -653,6 +652,7 @@ struct SurrArgs {
 };
 
 constexpr uint32_t TAG_SURR = 3u;
+constexpr int SURR_BATCH = 8;      // lanes that trigger one network evaluation
 
 // The four per-cell values a 1D event reads, staged once per block in LDS when the
 // table is small (one 16-byte LDS read per event instead of four HBM/L2 gathers).
@@ -792,10 +792,10 @@ __global__ __launch_bounds__(TPB) void k_transport(SoA s, Geo g, Scalars* sc, vo
           }
         }
       }
-      // Batch the network: evaluate once at least 16 lanes want it, or when no lane of the
+      // Batch the network: evaluate once SURR_BATCH lanes want it, or when no lane of the
       // wave has plain work left; until then wanting lanes wait (they do not advance).
       const int nwant = __popcll(__ballot(want));
-      if (nwant > 0 && (nwant >= 16 || nwant == __popcll(__ballot(has)))) {
+      if (nwant > 0 && (nwant >= SURR_BATCH || nwant == __popcll(__ballot(has)))) {
         mlp_forward_wave(s_w, s_b, s_scr + (threadIdx.x >> 6), want, feat, sout);
       } else if (want) {
         want = false;
@@ -857,6 +857,8 @@ __global__ __launch_bounds__(TPB) void k_transport(SoA s, Geo g, Scalars* sc, vo
       } else {                                                // leaves through a face
         const bool right = u24(q.r[1]) < sout[1];
         const T path = Tm<T>::from((double)(sout[2] * s_SB));
+        // the expected deposit E[1 - exp(-f sigma_a s) | exit] is a network output: weighting by
+        // exp(-f sigma_a E[s]) instead would overstate it (the map is concave in s)
         const W Enew = E * (W)(1.0f - sout[3]);
         val = (Acc)((E - Enew) * idxw[ix]);
         E = Enew;

@@ -97,12 +97,16 @@ __device__ __forceinline__ void mlp_forward_wave(const uint16_t* __restrict__ wb
   const float* b1 = bias;
   const float* b2 = bias + 64;
   const float* b3 = bias + 128;
-  for (int c = 0; c < 4; ++c) {
-    if (((mask >> (16 * c)) & 0xFFFFull) == 0ull) continue;   // wave-uniform
-    if (grp == c) {
-      const int sidx = l & 15;
+  // Active lanes are packed by rank into ceil(n/16) passes of 16 samples each.
+  const int nact = __popcll(mask);
+  const uint64_t lt = l == 0 ? 0ull : (~0ull >> (64 - l));
+  const int rank = __popcll(mask & lt);
+  for (int c = 0; 16 * c < nact; ++c) {                       // wave-uniform trip count
+    const bool mine = active && (rank >> 4) == c;
+    if (mine) {
+      const int sidx = rank & 15;
 #pragma unroll
-      for (int j = 0; j < 16; ++j) sc->xs[sidx * 16 + j] = (active && j < MLP_IN) ? f2bf(feat[j]) : (uint16_t)0;
+      for (int j = 0; j < 16; ++j) sc->xs[sidx * 16 + j] = j < MLP_IN ? f2bf(feat[j]) : (uint16_t)0;
     }
     wave_lds_sync();
     const mshort4 bx = frag_c(sc->xs, 16, 0, 0);             // B[k][s] = X[s][k]
@@ -132,8 +136,8 @@ __device__ __forceinline__ void mlp_forward_wave(const uint16_t* __restrict__ wb
       for (int r = 0; r < 4; ++r) sc->y[(l & 15) * 4 + r] = acc[r] + b3[r];
     }
     wave_lds_sync();
-    if (grp == c && active) {
-      const int sidx = l & 15;
+    if (mine) {
+      const int sidx = rank & 15;
 #pragma unroll
       for (int o = 0; o < MLP_OUT; ++o) out[o] = sigmoidf_(sc->y[sidx * 4 + o]);
     }

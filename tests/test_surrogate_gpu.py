@@ -171,7 +171,7 @@ def test_surrogate_transport_reproduces_plain_tallies():
         inp = _c4_inputs()
         mesh, sv, parts = M.setup(inp, seed=700 + sd)
         if params is None:
-            info = Surrogate.train(parts, n_samples=10_000_000, epochs=12, batch=8192, lr=3e-3, seed=2)
+            info = Surrogate.train(parts, seed=2)
             params = parts.surrogate_params()
             print("trained", info["loss_per_epoch"], info["generate_s"], info["train_s"])
         else:
