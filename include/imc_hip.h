@@ -67,6 +67,7 @@ typedef struct {
   int32_t transport_tally_mode;   /* 0 wave rows, 1 block row (LDS atomics), 2 global atomics */
   int32_t transport_grid;         /* persistent transport blocks of the last launch */
   int64_t surrogate_events;       /* thick-cell walks replaced by the surrogate */
+  int64_t truncated;              /* histories ended by the 2^24-event cap (non-finite state) */
 } imc_timing;
 
 /* Creates a context: MixedPrecisionIMC.jl:117 (particles = Vector{Vector{}}()) plus the

@@ -67,6 +67,7 @@ class ImcTiming(ctypes.Structure):
         ("transport_tally_mode", ctypes.c_int32),
         ("transport_grid", ctypes.c_int32),
         ("surrogate_events", ctypes.c_int64),
+        ("truncated", ctypes.c_int64),
     ]
 
 

@@ -134,6 +134,7 @@ Geo make_geo(const Ctx& c) {
   g.isig = c.d_isig; g.idxw = c.d_idxw; g.idyw = c.d_idyw;
   g.ics = 1.0 / (c.cfg.phys_c * c.cfg.distancescale);
   g.cell_off = -1;
+  g.gcopies = gtally_copies(c.NB);
   for (int k = 0; k < 16; ++k) g.scales[k] = c.cfg.energyscales[k];
   return g;
 }
@@ -203,6 +204,7 @@ int imc_create(imc_ctx** out, const imc_config* cfg) {
   ok &= dalloc(&c.d_red, 8 * (size_t)c.red_len) == hipSuccess;
   ok &= dalloc(&c.d_emitted, 8 * C * c.K) == hipSuccess;
   ok &= dalloc(&c.d_dep, 8 * (size_t)c.NB) == hipSuccess;
+  ok &= dalloc(&c.d_gtally, 8 * (size_t)c.NB * gtally_copies(c.NB)) == hipSuccess;
   ok &= dalloc(&c.d_rad, 8 * C) == hipSuccess;
   ok &= dalloc(&c.d_sc, sizeof(Scalars)) == hipSuccess;
   if (!ok) return fail(c, IMC_ENOMEM, "device allocation failed");
@@ -231,7 +233,7 @@ int imc_destroy(imc_ctx* h) {
   free_soa(c.buf[0]); free_soa(c.buf[1]);
   void* ptrs[] = {c.d_dx, c.d_dy, c.d_dxs, c.d_dys, c.d_dxw, c.d_dyw, c.d_temp, c.d_fleck, c.d_sa, c.d_ss,
                   c.d_radsrc, c.d_tsurf, c.d_sigcol, c.d_sigdep, c.d_isig, c.d_idxw, c.d_idyw, c.d_ent_e, c.d_ent_esc, c.d_ent_term,
-                  c.d_ent_k, c.d_ent_cnt, c.d_ent_off, c.d_red, c.d_emitted, c.d_dep, c.d_rad, c.d_sc,
+                  c.d_ent_k, c.d_ent_cnt, c.d_ent_off, c.d_red, c.d_emitted, c.d_dep, c.d_gtally, c.d_rad, c.d_sc,
                   c.d_partial, c.d_tile, c.d_bhist, c.d_surr_feat, c.d_surr_targ, c.d_surr_params, c.d_surr_grad,
                   c.d_surr_m, c.d_surr_v, c.d_surr_wbf, c.d_surr_bias, c.d_surr_loss, c.d_surr_events};
   for (void* p : ptrs) (void)hipFree(p);
@@ -351,11 +353,12 @@ int imc_transport(imc_ctx* h, double* energydep, double* lostenergy, int64_t* se
   HIPCHK(hipEventRecord(c.ev0, c.stream), "event");
   HIPCHK(launch_zero(c, (char*)c.d_sc + offsetof(Scalars, segments), sizeof(Scalars) - offsetof(Scalars, segments)),
          "zero scalars");
-  if (tm == TM_GLOBAL) HIPCHK(launch_zero(c, c.d_dep, 8 * (size_t)c.NB), "zero tallies");
+  if (tm == TM_GLOBAL) HIPCHK(launch_zero(c, c.d_gtally, 8 * (size_t)c.NB * gtally_copies(c.NB)), "zero tallies");
   HIPCHK(hipEventRecord(c.evk0, c.stream), "event");
   HIPCHK(launch_transport(c, grid, tm, lds), "transport kernel");
   HIPCHK(hipEventRecord(c.evk1, c.stream), "event");
-  HIPCHK(launch_reduce_partials(c, grid, tm, c.NB, c.d_dep), "tally reduction");
+  if (tm == TM_GLOBAL) HIPCHK(launch_sum_copies(c, c.d_gtally, gtally_copies(c.NB), c.NB, c.d_dep), "tally copies");
+  else HIPCHK(launch_reduce_partials(c, grid, tm, c.NB, c.d_dep), "tally reduction");
   if (energydep)
     HIPCHK(hipMemcpyAsync(energydep, c.d_dep, 8 * (size_t)c.C * c.K, hipMemcpyDeviceToHost, c.stream), "energydep D2H");
   double lost = 0.0;
@@ -370,6 +373,7 @@ int imc_transport(imc_ctx* h, double* energydep, double* lostenergy, int64_t* se
   c.timing.census_out += (int64_t)c.h_sc.census_out;
   c.timing.segments += (int64_t)c.h_sc.segments;
   c.timing.surrogate_events += (int64_t)c.h_sc.surr_events;
+  c.timing.truncated += (int64_t)c.h_sc.truncated;
   c.timing.transport_tally_mode = tm;
   c.timing.transport_grid = grid;
   if (lostenergy) *lostenergy = lost;

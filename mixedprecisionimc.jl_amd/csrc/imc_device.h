@@ -50,6 +50,7 @@ template <> struct Prec<16> { using T = _Float16; using W = float; using Cc = fl
 
 template <class T> struct Tm;
 template <> struct Tm<double> {
+  static __device__ __forceinline__ double maxval() { return 1.7976931348623157e308; }
   static __device__ __forceinline__ double from(double x) { return x; }
   static __device__ __forceinline__ double d(double x) { return x; }
   static __device__ __forceinline__ double exp_(double x) { return exp(x); }
@@ -68,6 +69,7 @@ template <> struct Tm<double> {
   }
 };
 template <> struct Tm<float> {
+  static __device__ __forceinline__ double maxval() { return 3.4028234663852886e38; }
   static __device__ __forceinline__ float from(double x) { return (float)x; }
   static __device__ __forceinline__ double d(float x) { return (double)x; }
   static __device__ __forceinline__ float exp_(float x) { return expf(x); }
@@ -114,6 +116,7 @@ __device__ __forceinline__ _Float16 f2h(float f) {
 
 template <> struct Tm<_Float16> {
   using h = _Float16;
+  static __device__ __forceinline__ double maxval() { return 65504.0; }
   static __device__ __forceinline__ h from(double x) { return f2h((float)x); }
   static __device__ __forceinline__ double d(h x) { return (double)(float)x; }
   static __device__ __forceinline__ h exp_(h x) { return f2h(expf((float)x)); }

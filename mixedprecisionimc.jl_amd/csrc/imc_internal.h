@@ -8,6 +8,16 @@
 
 namespace imc {
 
+// The global-atomic transport tally is spread over copies (block b adds into copy
+// b mod ncopies) so that hot bins do not serialise on one address; the copies are summed in
+// a fixed order after the launch.  ncopies = min(GTALLY_MAX_COPIES, GTALLY_BUDGET / (8 NB)).
+constexpr int GTALLY_MAX_COPIES = 64;
+constexpr size_t GTALLY_BUDGET = (size_t)256 << 20;
+inline int gtally_copies(int64_t NB) {
+  const int64_t k = (int64_t)(GTALLY_BUDGET / (8 * (size_t)NB));
+  return (int)(k < 1 ? 1 : (k > GTALLY_MAX_COPIES ? GTALLY_MAX_COPIES : k));
+}
+
 // Structure-of-arrays particle store in HBM.  Element types follow the precision:
 // x/y/mu(theta)/t in T, E/E0 in W, cellk = linear cell | scale index << 28 (DEAD if killed).
 struct SoA {
@@ -32,7 +42,7 @@ struct Scalars {
   unsigned long long census_out;
   unsigned long long surr_events;  // surrogate evaluations of the last transport
   uint32_t work;               // transport work-queue cursor
-  uint32_t pad;
+  uint32_t truncated;          // histories ended by the per-history event cap
 };
 
 struct SurrBox {               // training box of the thick-cell surrogate (see imc_mlp.h)
@@ -58,6 +68,7 @@ struct Geo {                   // kernel view of the geometry and step coefficie
   const void* idyw;            // W: 1/dy per row (2D)
   double ics;                  // 1/(phys_c * distancescale)
   int cell_off;                // 1D: byte offset of the LDS cell table, -1 when cells are read from HBM
+  int gcopies;                 // rows of the global-atomic tally
   double scales[16];
 };
 
@@ -102,6 +113,7 @@ struct Ctx {
   void* d_partial = nullptr;       // per-block partial sums
   size_t partial_bytes = 0;
   double* d_dep = nullptr;         // NB = C*K + 1 (last bin: lost energy)
+  double* d_gtally = nullptr;      // global-atomic tally: gtally_copies(NB) rows of NB bins
   double* d_rad = nullptr;         // C
   int64_t* d_tile = nullptr;       // clean: per-tile counts/offsets
   int64_t* d_bhist = nullptr;      // births: per-tile start-time bucket counts/offsets
@@ -143,6 +155,7 @@ hipError_t launch_transport(Ctx& c, int grid_blocks, int tally_mode, size_t lds_
 hipError_t launch_clean(Ctx& c, int64_t n_total);
 hipError_t launch_reduce_partials(Ctx& c, int nblocks, int tally_mode, int nbins, double* out);
 hipError_t launch_zero(Ctx& c, void* p, size_t bytes);
+hipError_t launch_sum_copies(Ctx& c, const double* copies, int ncopies, int64_t n, double* out);
 int transport_choose(const Ctx& c, int* grid_blocks, int* tally_mode, size_t* lds_bytes);
 int clean_choose(const Ctx& c, int* grid_blocks, int* tally_mode, size_t* lds_bytes);
 hipError_t surr_generate(Ctx& c, int64_t n, uint32_t seed, unsigned long long* d_events);

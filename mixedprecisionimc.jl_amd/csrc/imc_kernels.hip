@@ -34,6 +34,21 @@ __global__ void k_fill_u64(unsigned long long* p, size_t n) {
     p[i] = 0ull;
 }
 
+// out[j] = sum over copies of copies[k][j], in copy order.
+__global__ void k_sum_copies(const double* __restrict__ copies, int ncopies, int64_t n, double* __restrict__ out) {
+  for (int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; j < n; j += (int64_t)gridDim.x * blockDim.x) {
+    double sum = 0.0;
+    for (int k = 0; k < ncopies; ++k) sum += copies[(size_t)k * n + j];
+    out[j] = sum;
+  }
+}
+
+hipError_t launch_sum_copies(Ctx& c, const double* copies, int ncopies, int64_t n, double* out) {
+  const int grid = (int)std::min<int64_t>((n + 255) / 256, 4096);
+  k_sum_copies<<<grid, 256, 0, c.stream>>>(copies, ncopies, n, out);
+  return hipGetLastError();
+}
+
 hipError_t launch_zero(Ctx& c, void* p, size_t bytes) {
   if (bytes == 0) return hipSuccess;
   if ((bytes & 7) == 0 && (((uintptr_t)p) & 7) == 0) {
@@ -64,7 +79,10 @@ __global__ void k_prep(int nx, int ny, int C, double s, const double* __restrict
   if (i < C) {
     const double sc = sa[i] * (1.0 - f[i]) + ss[i];
     sigcol[i] = Tm<T>::from(sc);                              // imc_transport.jl:87
-    isig[i] = Tm<T>::from(1.0 / sc);                          // :87 as a product
+    // :87 as a product.  The reciprocal is clamped to the largest finite T: an infinite one
+    // (sigma_c = 0, or 1/sigma_c beyond binary16's 65504) would make 0 * inf = NaN when the
+    // exponential variate is 0, where the quotient gives 0.
+    isig[i] = Tm<T>::from(fmin(1.0 / sc, Tm<T>::maxval()));
     sigdep[i] = (W)(sa[i] * f[i]);                            // imc_transport.jl:95
   }
   if (i < nx) { dxs[i] = Tm<T>::from(dx[i] * s); dxw[i] = (W)dx[i]; idxw[i] = (W)(1.0 / dx[i]); }
@@ -653,6 +671,10 @@ struct SurrArgs {
 
 constexpr uint32_t TAG_SURR = 3u;
 constexpr int SURR_BATCH = 8;      // lanes that trigger one network evaluation
+// Event cap per history.  A history still running after EV_MAX events (only a non-finite
+// state can do that) is ended like the 1% cut-off and counted in Scalars::truncated, so
+// every lane, and with it every wave, finishes.
+constexpr uint32_t EV_MAX = 1u << 24;
 
 // The four per-cell values a 1D event reads, staged once per block in LDS when the
 // table is small (one 16-byte LDS read per event instead of four HBM/L2 gathers).
@@ -697,7 +719,7 @@ __global__ __launch_bounds__(TPB) void k_transport(SoA s, Geo g, Scalars* sc, vo
     s_cell = w;
   }
   tally_init<Acc, TM>(lds, NB);       // its barrier also publishes the cell table
-  Tally<Acc, TM> tal{lds, gbins, NB};
+  Tally<Acc, TM> tal{lds, TM == TM_GLOBAL ? gbins + (size_t)(blockIdx.x % g.gcopies) * NB : gbins, NB};
 
   const T* __restrict__ dxs = (const T*)g.dxs;
   const T* __restrict__ dys = (const T*)g.dys;
@@ -733,6 +755,7 @@ __global__ __launch_bounds__(TPB) void k_transport(SoA s, Geo g, Scalars* sc, vo
   uint32_t cur = 0, cend = 0;
   bool done = false;
   unsigned long long segs = 0, cens = 0, nsurr = 0;
+  uint32_t ntrunc = 0;
 
   for (;;) {
     // ---- lane refill from the wave's chunk ----
@@ -932,7 +955,9 @@ __global__ __launch_bounds__(TPB) void k_transport(SoA s, Geo g, Scalars* sc, vo
       const W Enew = E * Tm<W>::exp_(-tau);                                   // :95 / :580
       const W Evol = TWO_D ? (E * cidx) * idyw[iy] : E * cidx;
       bin = k * C + cell;
-      if (Enew <= Emin) {                                                     // :97-106 / :586-595
+      const bool capped = ev >= EV_MAX;
+      ntrunc += capped ? 1u : 0u;
+      if (Enew <= Emin || capped) {                                           // :97-106 / :586-595
         val = (Acc)Evol;
         CK[slot] = DEAD;
         has = false;
@@ -1028,9 +1053,11 @@ __global__ __launch_bounds__(TPB) void k_transport(SoA s, Geo g, Scalars* sc, vo
   segs = wave_sum(segs);
   cens = wave_sum(cens);
   if (SURR) nsurr = wave_sum(nsurr);
+  ntrunc = (uint32_t)wave_sum((unsigned long long)ntrunc);
   if (lane == 0) {
     atomicAdd(&sc->segments, segs);
     atomicAdd(&sc->census_out, cens);
+    if (ntrunc) atomicAdd(&sc->truncated, ntrunc);
     if (SURR) atomicAdd(&sc->surr_events, nsurr);
   }
   tally_flush<Acc, TM>(lds, NB, (Acc*)partial_v);
@@ -1157,7 +1184,7 @@ hipError_t launch_transport(Ctx& c, int grid, int tm, size_t lds) {
   sa.tally_bytes = (int)lds;
   const size_t total = lds + (use_surrogate(c) ? SURR_LDS : 0);
 #define TR(P, TM_, D, S, L) \
-  k_transport<P, TM_, D, S, L><<<grid, TPB, total, c.stream>>>(s, g, c.d_sc, c.d_partial, c.d_dep, sa)
+  k_transport<P, TM_, D, S, L><<<grid, TPB, total, c.stream>>>(s, g, c.d_sc, c.d_partial, c.d_gtally, sa)
   IMC_TRANSPORT_DISPATCH(TR);
 #undef TR
   return hipGetLastError();

@@ -28,6 +28,9 @@ namespace {
 
 constexpr uint32_t TAG_BIRTH = 1u;
 constexpr uint32_t TAG_EVENT = 2u;
+// A history still running after this many events (only a non-finite state can) ends like
+// the 1% cut-off, as in the engine; the reference would loop forever.
+constexpr uint32_t ORC_EV_MAX = 1u << 24;
 constexpr double PI_D = 3.141592653589793;
 constexpr double TWO_PI_D = 6.283185307179586;
 
@@ -95,6 +98,7 @@ inline bool operator!=(h16 a, h16 b) { return a.v != b.v; }
 
 template <class T> struct Tr;
 template <> struct Tr<double> {
+  static double maxval() { return 1.7976931348623157e308; }   // largest finite value
   static double from(double x) { return x; }
   static double d(double x) { return x; }
   static double exp(double x) { return std::exp(x); }
@@ -114,6 +118,7 @@ template <> struct Tr<double> {
   static double uscat(const U4& q, bool first) { return first ? u53(q.r[2], q.r[3]) : u53(q.r[0], q.r[1]); }
 };
 template <> struct Tr<float> {
+  static double maxval() { return 3.4028234663852886e38; }
   static float from(double x) { return (float)x; }
   static double d(float x) { return (double)x; }
   static float exp(float x) { return expf(x); }
@@ -129,6 +134,7 @@ template <> struct Tr<float> {
   static float uscat(const U4& q, bool first) { return first ? u24(q.r[1]) : u24(q.r[0]); }
 };
 template <> struct Tr<h16> {
+  static double maxval() { return 65504.0; }
   static h16 from(double x) { return h16((float)x); }
   static double d(h16 x) { return (double)x.v; }
   static h16 exp(h16 x) { return h16(expf(x.v)); }
@@ -432,7 +438,7 @@ void transport1d_t(const orc_problem* pb, const orc_step* st, uint32_t step, dou
   for (int i = 0; i < C; ++i) {
     const double sa = st->sigma_a[i], f = st->fleck[i], ss = st->sigma_s[i];
     dxs[i] = Tr<T>::from(st->dx[i] * pb->distancescale);
-    isig[i] = Tr<T>::from(1.0 / (sa * (1.0 - f) + ss));    // :87 1/(effective scattering)
+    isig[i] = Tr<T>::from(std::min(1.0 / (sa * (1.0 - f) + ss), Tr<T>::maxval()));    // :87 1/(effective scattering)
     sigdep[i] = (W)(sa * f);                                // :95 / :120 sigma_a*f
     idxw[i] = (W)(1.0 / st->dx[i]);                         // :104 / :120 1/dx
   }
@@ -467,7 +473,7 @@ void transport1d_t(const orc_problem* pb, const orc_step* st, uint32_t step, dou
       const T d = jmin(jmin(db, dc), dcen);                 // :92
       const W tau = sigdep[i] * (W)Tr<T>::d(d);
       const W Enew = E * (W)Tr<W>::exp(-tau);               // :95
-      if (Enew <= Emin) {                                   // :97-106
+      if (Enew <= Emin || ev >= ORC_EV_MAX) {              // :97-106; event cap as the engine
         D[k * C + i] += (double)(E * idxw[i]);
         r[7] = -1.0;
         break;
@@ -532,7 +538,7 @@ void transport2d_t(const orc_problem* pb, const orc_step* st, uint32_t step, dou
   for (int j = 0; j < ny; ++j) { dys[j] = Tr<T>::from(st->dy[j] * pb->distancescale); idyw[j] = (W)(1.0 / st->dy[j]); }
   for (int c = 0; c < C; ++c) {
     const double sa = st->sigma_a[c], f = st->fleck[c], ss = st->sigma_s[c];
-    isig[c] = Tr<T>::from(1.0 / (sa * (1.0 - f) + ss));    // :561
+    isig[c] = Tr<T>::from(std::min(1.0 / (sa * (1.0 - f) + ss), Tr<T>::maxval()));    // :561
     sigdep[c] = (W)(sa * f);                                // :580 / :607
   }
   const T cT = Tr<T>::from(pb->phys_c), sT = Tr<T>::from(pb->distancescale), dtT = Tr<T>::from(st->dt);
@@ -573,7 +579,7 @@ void transport2d_t(const orc_problem* pb, const orc_step* st, uint32_t step, dou
       const T d = jmin(jmin(db, dc), dcen);                 // :571
       const W tau = sigdep[c] * (W)Tr<T>::d(d);
       const W Enew = E * (W)Tr<W>::exp(-tau);               // :580
-      if (Enew <= Emin) {                                   // :586-595
+      if (Enew <= Emin || ev >= ORC_EV_MAX) {              // :586-595; event cap as the engine
         D[k * C + c] += (double)((E * idxw[ix]) * idyw[iy]);
         r[7] = -1.0;
         break;

@@ -40,12 +40,12 @@ def case2d(nx=12, ny=9, prec=64, seed=13, bc=(0, 1, 1, 0), scales=(1.0,), dt=0.0
                 tsurf=tsurf)
 
 
-def engine(case, capacity=1 << 16, quirks=0):
+def engine(case, capacity=1 << 16, quirks=0, **tuning):
     from MixedPrecisionIMC._hip import Engine
 
     e = Engine(geometry=case["geometry"], precision=case["prec"], nx=case["nx"], ny=case["ny"],
                energyscales=case["scales"], bc=case["bc"], seed=case["seed"], phys_c=case["phys_c"],
-               phys_a=case["phys_a"], capacity=capacity, quirks=quirks)
+               phys_a=case["phys_a"], capacity=capacity, quirks=quirks, **tuning)
     e.set_geometry(case["dx"], case["dy"])
     return e
 

@@ -55,8 +55,8 @@ def test_source_matches_oracle(prec, geom):
     e.close()
 
 
-def _replay(case, n_input=3000, step=2):
-    e = engine(case)
+def _replay(case, n_input=3000, step=2, **tuning):
+    e = engine(case, **tuning)
     op, st = oracle_problem(case)
     rec, _, _, _ = op.source(st, step, None, n_input, 10 * n_input, 2.0)
     engine_step(e, case, step)
@@ -129,4 +129,45 @@ def test_multi_scale_and_reflect(prec):
     total = np.abs(o_dep).sum()
     tol = 1e-9 if prec == 64 else 2e-3
     assert np.abs(dep - o_dep).sum() <= tol * total
+    e.close()
+
+
+@pytest.mark.parametrize("prec", [32, 16])
+def test_zero_variate_in_a_transparent_cell_gpu(prec):
+    """GPU counterpart of test_oracle_pins.test_zero_variate_in_a_transparent_cell_stays_finite:
+    a first event with exponential variate 0 in a cell whose 1/sigma_c overflows the tracking
+    type ends finite, at census, like the oracle's history."""
+    from test_oracle_pins import transparent_case, transparent_record
+    case = transparent_case(prec)
+    print(f"[zero variate] prec={prec}", flush=True)
+    e = engine(case)
+    engine_step(e, case, 0)
+    rec = transparent_record()
+    e.import_particles(rec)
+    dep, lost, segs = e.transport()
+    g = e.export_particles()
+    op, st = oracle_problem(case)
+    o_rec, o_dep, o_lost, o_segs = op.transport(st, 0, rec)
+    assert np.all(np.isfinite(g)) and np.all(np.isfinite(dep)) and np.isfinite(lost)
+    assert e.timing()["truncated"] == 0
+    assert g[0, 7] != -1.0 and segs == o_segs
+    assert g[0, 2] == o_rec[0, 2]
+    assert g[0, 6] == pytest.approx(o_rec[0, 6], rel=1e-3)
+    e.close()
+
+
+@pytest.mark.parametrize("mode", [1, 2, 3])
+@pytest.mark.parametrize("geom", [1, 2])
+def test_transport_replay_every_tally_mode(mode, geom):
+    """Per-wave LDS rows (1), one LDS row per block (2) and replicated global FP64 atomics (3)
+    give the oracle's tallies at FP64; blocks_per_cu=2 also exercises a small persistent grid."""
+    case = case1d(prec=64) if geom == 1 else case2d(prec=64)
+    print(f"[tally mode] mode={mode} geometry={geom}", flush=True)
+    e, op, st, rec, (dep, lost, segs, g), (o_dep, o_lost, o_segs, o_rec) = _replay(case, tally_mode=mode,
+                                                                                     blocks_per_cu=2)
+    tm = e.timing()["transport_tally_mode"]
+    assert tm == {1: 0, 2: 1, 3: 2}[mode]
+    np.testing.assert_allclose(dep, o_dep, rtol=1e-9, atol=1e-9 * np.abs(o_dep).max())
+    assert lost == pytest.approx(o_lost, rel=1e-9, abs=1e-15)
+    assert np.count_nonzero(_alive(g) != _alive(o_rec)) == 0
     e.close()

@@ -228,3 +228,35 @@ def test_tree_sum_is_a_sum():
         [case["tsurf"][0][0] ** 4 * dt / 4, case["tsurf"][1][0] ** 4 * dt / 4]
     exact = float(sum(Fraction(x) for x in terms))
     assert etot == pytest.approx(exact, rel=1e-13)
+
+
+# Seed whose Philox block (slot 0, event 0, TAG_EVENT, 0) keyed (seed, step 0) has word 0 =
+# 0xFFFFFF23: its 24-bit uniform is exactly 1, so the exponential variate -log(u) is 0.
+ZERO_VARIATE_SEED = 56716214
+
+
+def transparent_case(prec):
+    """Three-cell reflecting box whose 1/sigma_c does not fit the tracking type: sigma_a = 1e-7
+    (1/sigma_c = 2e7 > 65504) at FP16, sigma_a = 0 at FP32."""
+    sa = 1e-7 if prec == 16 else 0.0
+    return dict(geometry=1, prec=prec, nx=3, ny=1, seed=ZERO_VARIATE_SEED, bc=(1, 1, 0, 0), scales=(1.0,), dt=1.0,
+                phys_c=1.0, phys_a=1.0, dx=np.full(3, 0.1), dy=None, sa=np.full(3, sa), ss=np.zeros(3),
+                temp=np.ones(3), fleck=np.full(3, 0.5), radsource=np.zeros(3), tsurf=[np.zeros(1), np.zeros(1)])
+
+
+def transparent_record():
+    return np.array([[1.0, 0.0, 2.0, 0.05, 0.5, 1.0, 1.0, 1.0, 1.0]])
+
+
+@pytest.mark.parametrize("prec", [32, 16])
+def test_zero_variate_in_a_transparent_cell_stays_finite(prec):
+    """The collision distance is variate * (1/sigma_c) with the reciprocal clamped to the
+    largest finite value of the tracking type; unclamped, 0 * inf = NaN would make the
+    history run forever.  Here the first event draws variate 0 in such a cell."""
+    assert O.philox([0, 0, 2, 0], [ZERO_VARIATE_SEED, 0])[0] >> 8 == 0xFFFFFF
+    case = transparent_case(prec)
+    op, st = oracle_problem(case)
+    out, dep, lost, segs = op.transport(st, 0, transparent_record())
+    assert np.all(np.isfinite(out)) and np.all(np.isfinite(dep)) and np.isfinite(lost)
+    assert out[0, 7] != -1.0 and segs < 100          # streams to census (path 1 = 10 cells)
+    assert out[0, 6] == pytest.approx(1.0, rel=1e-3)
