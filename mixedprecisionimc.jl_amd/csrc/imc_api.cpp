@@ -300,8 +300,15 @@ int imc_set_step_fields(imc_ctx* h, int64_t step, double dt, const double* sigma
   HIPCHK(hipMemcpyAsync(c.d_temp, temp, 8 * C, hipMemcpyHostToDevice, c.stream), "temp H2D");
   HIPCHK(hipMemcpyAsync(c.d_radsrc, radsource, 8 * C, hipMemcpyHostToDevice, c.stream), "radsource H2D");
   HIPCHK(hipMemcpyAsync(c.d_tsurf, ts.data(), 8 * ts.size(), hipMemcpyHostToDevice, c.stream), "tsurf H2D");
+  HIPCHK(launch_zero(c, (char*)c.d_sc + offsetof(Scalars, prep_underflow), sizeof(uint32_t)), "zero prep flag");
   HIPCHK(launch_prep(c), "prep kernel");
+  uint32_t underflow = 0;
+  HIPCHK(hipMemcpyAsync(&underflow, (char*)c.d_sc + offsetof(Scalars, prep_underflow), sizeof(uint32_t),
+                        hipMemcpyDeviceToHost, c.stream), "prep flag D2H");
   HIPCHK(hipStreamSynchronize(c.stream), "step fields sync");
+  if (underflow)
+    return fail(c, IMC_ENUMERIC, (std::to_string(underflow) + " cell(s) have a collision mean free path "
+                "1/sigma_c that rounds to 0 in the tracking precision; use a wider PRECISION").c_str());
   c.step_set = true;
   return IMC_OK;
 }

@@ -43,6 +43,8 @@ struct Scalars {
   unsigned long long surr_events;  // surrogate evaluations of the last transport
   uint32_t work;               // transport work-queue cursor
   uint32_t truncated;          // histories ended by the per-history event cap
+  uint32_t prep_underflow;     // cells whose 1/sigma_c rounds to 0 in the tracking type
+  uint32_t pad2;
 };
 
 struct SurrBox {               // training box of the thick-cell surrogate (see imc_mlp.h)

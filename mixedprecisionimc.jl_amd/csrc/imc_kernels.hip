@@ -71,7 +71,8 @@ __global__ void k_prep(int nx, int ny, int C, double s, const double* __restrict
                        typename Prec<P>::T* dxs, typename Prec<P>::T* dys,
                        typename Prec<P>::W* dxw, typename Prec<P>::W* dyw,
                        typename Prec<P>::T* sigcol, typename Prec<P>::W* sigdep,
-                       typename Prec<P>::T* isig, typename Prec<P>::W* idxw, typename Prec<P>::W* idyw) {
+                       typename Prec<P>::T* isig, typename Prec<P>::W* idxw, typename Prec<P>::W* idyw,
+                       Scalars* sc_out) {
 #pragma clang fp contract(off)
   using T = typename Prec<P>::T;
   using W = typename Prec<P>::W;
@@ -82,7 +83,10 @@ __global__ void k_prep(int nx, int ny, int C, double s, const double* __restrict
     // :87 as a product.  The reciprocal is clamped to the largest finite T: an infinite one
     // (sigma_c = 0, or 1/sigma_c beyond binary16's 65504) would make 0 * inf = NaN when the
     // exponential variate is 0, where the quotient gives 0.
-    isig[i] = Tm<T>::from(fmin(1.0 / sc, Tm<T>::maxval()));
+    const T is = Tm<T>::from(fmin(1.0 / sc, Tm<T>::maxval()));
+    isig[i] = is;
+    // a mean free path that rounds to 0 would make every collision distance 0
+    if (sc > 0.0 && Tm<T>::d(is) == 0.0) atomicAdd(&sc_out->prep_underflow, 1u);
     sigdep[i] = (W)(sa[i] * f[i]);                            // imc_transport.jl:95
   }
   if (i < nx) { dxs[i] = Tm<T>::from(dx[i] * s); dxw[i] = (W)dx[i]; idxw[i] = (W)(1.0 / dx[i]); }
@@ -98,7 +102,7 @@ hipError_t launch_prep(Ctx& c) {
       c.nx, c.ny, c.C, s, c.d_dx, c.d_dy, c.d_sa, c.d_ss, c.d_fleck,                             \
       (Prec<P>::T*)c.d_dxs, (Prec<P>::T*)c.d_dys, (Prec<P>::W*)c.d_dxw, (Prec<P>::W*)c.d_dyw,     \
       (Prec<P>::T*)c.d_sigcol, (Prec<P>::W*)c.d_sigdep, (Prec<P>::T*)c.d_isig, (Prec<P>::W*)c.d_idxw,  \
-      (Prec<P>::W*)c.d_idyw)
+      (Prec<P>::W*)c.d_idyw, c.d_sc)
   if (c.prec == 64) PREP(64);
   else if (c.prec == 32) PREP(32);
   else PREP(16);
@@ -756,10 +760,20 @@ __global__ __launch_bounds__(TPB) void k_transport(SoA s, Geo g, Scalars* sc, vo
   bool done = false;
   unsigned long long segs = 0, cens = 0, nsurr = 0;
   uint32_t ntrunc = 0;
+  // FP32/FP16 events use two words of a Philox block each, so block (slot, ev/2) serves
+  // events ev and ev+1; it is kept in registers between them.  Lanes start histories only
+  // on even iterations, which keeps every lane's event parity equal to the iteration's and
+  // makes odd iterations skip Philox altogether.
+  constexpr bool PAIRED = (P != 64);
+  u4 qc;
+  qc.r[0] = qc.r[1] = qc.r[2] = qc.r[3] = 0u;
+  uint32_t qkey = 0xFFFFFFFFu;
+  uint32_t it = 0;
 
-  for (;;) {
-    // ---- lane refill from the wave's chunk ----
-    uint64_t need = __ballot(!has);
+  for (;; ++it) {
+    // ---- lane refill from the wave's chunk (even iterations only when PAIRED) ----
+    const bool refill_now = !PAIRED || (it & 1u) == 0u;
+    uint64_t need = refill_now ? __ballot(!has) : 0ull;
     while (need != 0ull && !done) {
       if (cur >= cend) {
         uint32_t base = 0;
@@ -782,12 +796,16 @@ __global__ __launch_bounds__(TPB) void k_transport(SoA s, Geo g, Scalars* sc, vo
         Emin = (W)(0.01 * (double)E0[slot]);                 // :61 / :531
         if (TWO_D) { ox = Tm<T>::cos_(mu); oy = Tm<T>::sin_(mu); }
         ev = 0;
+        qkey = 0xFFFFFFFFu;
         has = true;
       }
       cur += min((uint32_t)__popcll(need), avail);
       need = __ballot(!has);
     }
-    if (!__any(has)) break;
+    if (!__any(has)) {
+      if (done || refill_now) break;     // nothing left, or the refill found no work
+      continue;                          // odd iteration of an idle wave: refill next
+    }
 
     int bin = -1, lbin = -1;
     Acc val = (Acc)0, lval = (Acc)0;
@@ -926,7 +944,16 @@ __global__ __launch_bounds__(TPB) void k_transport(SoA s, Geo g, Scalars* sc, vo
       ++ev;
     } else if (has) {
       ++segs;
-      const u4 q = philox(slot, ev, TAG_EVENT, 0u, g.seed, g.step);
+      u4 q;
+      if (!PAIRED) {
+        q = philox(slot, ev, TAG_EVENT, 0u, g.seed, g.step);
+      } else {
+        if ((ev >> 1) != qkey) { qc = philox(slot, ev >> 1, TAG_EVENT, 0u, g.seed, g.step); qkey = ev >> 1; }
+        const bool second = (ev & 1u) != 0u;
+        q.r[0] = second ? qc.r[2] : qc.r[0];    // collision distance
+        q.r[1] = second ? qc.r[3] : qc.r[1];    // scattering direction
+        q.r[2] = 0u; q.r[3] = 0u;
+      }
       const int cell = TWO_D ? ix + nx * iy : ix;
       T cdx, cis;
       W csd, cidx;

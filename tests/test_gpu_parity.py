@@ -171,3 +171,20 @@ def test_transport_replay_every_tally_mode(mode, geom):
     assert lost == pytest.approx(o_lost, rel=1e-9, abs=1e-15)
     assert np.count_nonzero(_alive(g) != _alive(o_rec)) == 0
     e.close()
+
+
+def test_fp16_rejects_a_mean_free_path_it_cannot_represent():
+    """sigma_c = 1e9: 1/sigma_c rounds to 0 in binary16, so every collision distance would be 0;
+    the step is refused with IMC_ENUMERIC instead.  FP32 represents it and accepts the step."""
+    from MixedPrecisionIMC._hip import ImcError
+    for prec, ok in ((16, False), (32, True)):
+        case = case1d(prec=prec, C=8)
+        case["sa"] = case["sa"].copy()
+        case["sa"][3] = 1e9
+        e = engine(case)
+        if ok:
+            engine_step(e, case, 0)
+        else:
+            with pytest.raises(ImcError, match="mean free path"):
+                engine_step(e, case, 0)
+        e.close()
