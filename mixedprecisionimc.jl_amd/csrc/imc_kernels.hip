@@ -760,20 +760,10 @@ __global__ __launch_bounds__(TPB) void k_transport(SoA s, Geo g, Scalars* sc, vo
   bool done = false;
   unsigned long long segs = 0, cens = 0, nsurr = 0;
   uint32_t ntrunc = 0;
-  // FP32/FP16 events use two words of a Philox block each, so block (slot, ev/2) serves
-  // events ev and ev+1; it is kept in registers between them.  Lanes start histories only
-  // on even iterations, which keeps every lane's event parity equal to the iteration's and
-  // makes odd iterations skip Philox altogether.
-  constexpr bool PAIRED = (P != 64);
-  u4 qc;
-  qc.r[0] = qc.r[1] = qc.r[2] = qc.r[3] = 0u;
-  uint32_t qkey = 0xFFFFFFFFu;
-  uint32_t it = 0;
 
-  for (;; ++it) {
-    // ---- lane refill from the wave's chunk (even iterations only when PAIRED) ----
-    const bool refill_now = !PAIRED || (it & 1u) == 0u;
-    uint64_t need = refill_now ? __ballot(!has) : 0ull;
+  for (;;) {
+    // ---- lane refill from the wave's chunk ----
+    uint64_t need = __ballot(!has);
     while (need != 0ull && !done) {
       if (cur >= cend) {
         uint32_t base = 0;
@@ -796,16 +786,12 @@ __global__ __launch_bounds__(TPB) void k_transport(SoA s, Geo g, Scalars* sc, vo
         Emin = (W)(0.01 * (double)E0[slot]);                 // :61 / :531
         if (TWO_D) { ox = Tm<T>::cos_(mu); oy = Tm<T>::sin_(mu); }
         ev = 0;
-        qkey = 0xFFFFFFFFu;
         has = true;
       }
       cur += min((uint32_t)__popcll(need), avail);
       need = __ballot(!has);
     }
-    if (!__any(has)) {
-      if (done || refill_now) break;     // nothing left, or the refill found no work
-      continue;                          // odd iteration of an idle wave: refill next
-    }
+    if (!__any(has)) break;
 
     int bin = -1, lbin = -1;
     Acc val = (Acc)0, lval = (Acc)0;
@@ -944,16 +930,7 @@ __global__ __launch_bounds__(TPB) void k_transport(SoA s, Geo g, Scalars* sc, vo
       ++ev;
     } else if (has) {
       ++segs;
-      u4 q;
-      if (!PAIRED) {
-        q = philox(slot, ev, TAG_EVENT, 0u, g.seed, g.step);
-      } else {
-        if ((ev >> 1) != qkey) { qc = philox(slot, ev >> 1, TAG_EVENT, 0u, g.seed, g.step); qkey = ev >> 1; }
-        const bool second = (ev & 1u) != 0u;
-        q.r[0] = second ? qc.r[2] : qc.r[0];    // collision distance
-        q.r[1] = second ? qc.r[3] : qc.r[1];    // scattering direction
-        q.r[2] = 0u; q.r[3] = 0u;
-      }
+      const u4 q = philox(slot, ev, TAG_EVENT, 0u, g.seed, g.step);
       const int cell = TWO_D ? ix + nx * iy : ix;
       T cdx, cis;
       W csd, cidx;

@@ -19,7 +19,6 @@
 #include <chrono>
 #include <cmath>
 #include <cstring>
-#include <type_traits>
 #include <vector>
 #ifdef _OPENMP
 #include <omp.h>
@@ -54,18 +53,6 @@ inline U4 philox(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3, uint32_t k0
     c0 = n0; c1 = n1; c2 = n2; c3 = n3;
   }
   return U4{{c0, c1, c2, c3}};
-}
-
-// Words of event ev of the history in slot p.  FP64 takes a Philox block per event (two
-// 53-bit uniforms); FP32 and FP16 take words 2(ev&1) and 2(ev&1)+1 of block (p, ev/2), so
-// one block serves two consecutive events (word 0: collision distance, word 1: scattering).
-template <class T> inline U4 event_words(uint32_t p, uint32_t ev, uint32_t seed, uint32_t step) {
-  if (std::is_same<T, double>::value) return philox(p, ev, TAG_EVENT, 0u, seed, step);
-  const U4 b = philox(p, ev >> 1, TAG_EVENT, 0u, seed, step);
-  const int o = (ev & 1u) ? 2 : 0;
-  U4 q;
-  q.r[0] = b.r[o]; q.r[1] = b.r[o + 1]; q.r[2] = 0u; q.r[3] = 0u;
-  return q;
 }
 
 inline double u53(uint32_t hi, uint32_t lo) {            // [0,1), 53 bits
@@ -479,7 +466,7 @@ void transport1d_t(const orc_problem* pb, const orc_step* st, uint32_t step, dou
     int64_t segs = 0;
     for (uint32_t ev = 0;; ++ev) {
       ++segs;                                               // :73
-      const U4 q = event_words<T>((uint32_t)p, ev, seed, step);
+      const U4 q = philox((uint32_t)p, ev, TAG_EVENT, 0u, seed, step);
       const T db = (mu > zero) ? (dxs[i] - x) / mu : Tr<T>::abs(x / mu);   // :77-83
       const T dc = Tr<T>::expvar(q) * isig[i];              // :87
       const T dcen = cT * (dtT - t) * sT;                   // :89
@@ -578,7 +565,7 @@ void transport2d_t(const orc_problem* pb, const orc_step* st, uint32_t step, dou
     int64_t segs = 0;
     for (uint32_t ev = 0;; ++ev) {
       ++segs;
-      const U4 q = event_words<T>((uint32_t)p, ev, seed, step);
+      const U4 q = philox((uint32_t)p, ev, TAG_EVENT, 0u, seed, step);
       const int c = ix + nx * iy;
       const T ox = Tr<T>::cos(th), oy = Tr<T>::sin(th);     // :534
       const T dbx = (ox > zero) ? Tr<T>::abs((dxs[ix] - x) / ox) : Tr<T>::abs(x / ox);   // :538-542
