@@ -235,7 +235,7 @@ int imc_destroy(imc_ctx* h) {
                   c.d_radsrc, c.d_tsurf, c.d_sigcol, c.d_sigdep, c.d_isig, c.d_idxw, c.d_idyw, c.d_ent_e, c.d_ent_esc, c.d_ent_term,
                   c.d_ent_k, c.d_ent_cnt, c.d_ent_off, c.d_red, c.d_emitted, c.d_dep, c.d_gtally, c.d_rad, c.d_sc,
                   c.d_partial, c.d_tile, c.d_bhist, c.d_surr_feat, c.d_surr_targ, c.d_surr_params, c.d_surr_grad,
-                  c.d_surr_m, c.d_surr_v, c.d_surr_wbf, c.d_surr_bias, c.d_surr_loss, c.d_surr_events};
+                  c.d_surr_m, c.d_surr_v, c.d_surr_wbf, c.d_surr_bias, c.d_surr_loss, c.d_surr_events, c.d_surr_cells};
   for (void* p : ptrs) (void)hipFree(p);
   hipEvent_t evs[] = {c.ev0, c.ev1, c.evk0, c.evk1};
   for (hipEvent_t e : evs)
@@ -366,13 +366,13 @@ int imc_transport(imc_ctx* h, double* energydep, double* lostenergy, int64_t* se
   HIPCHK(hipEventRecord(c.evk1, c.stream), "event");
   if (tm == TM_GLOBAL) HIPCHK(launch_sum_copies(c, c.d_gtally, gtally_copies(c.NB), c.NB, c.d_dep), "tally copies");
   else HIPCHK(launch_reduce_partials(c, grid, tm, c.NB, c.d_dep), "tally reduction");
-  if (energydep)
-    HIPCHK(hipMemcpyAsync(energydep, c.d_dep, 8 * (size_t)c.C * c.K, hipMemcpyDeviceToHost, c.stream), "energydep D2H");
-  double lost = 0.0;
-  HIPCHK(hipMemcpyAsync(&lost, c.d_dep + (c.NB - 1), 8, hipMemcpyDeviceToHost, c.stream), "lost D2H");
+  c.h_dep.resize((size_t)c.NB);                 // C*K bins + lost energy, one copy
+  HIPCHK(hipMemcpyAsync(c.h_dep.data(), c.d_dep, 8 * (size_t)c.NB, hipMemcpyDeviceToHost, c.stream), "energydep D2H");
   HIPCHK(hipMemcpyAsync(&c.h_sc, c.d_sc, sizeof(Scalars), hipMemcpyDeviceToHost, c.stream), "scalars D2H");
   HIPCHK(hipEventRecord(c.ev1, c.stream), "event");
   HIPCHK(hipStreamSynchronize(c.stream), "transport sync");
+  if (energydep) std::memcpy(energydep, c.h_dep.data(), 8 * (size_t)c.C * c.K);
+  const double lost = c.h_dep[(size_t)c.NB - 1];
   c.timing.ms_transport += elapsed(c.ev0, c.ev1);
   c.timing.ms_transport_kernel += elapsed(c.evk0, c.evk1);
   c.timing.n_transport_launches += 1;
@@ -395,6 +395,10 @@ int imc_clean(imc_ctx* h, int64_t* n_alive) {
   if (!c.step_set) return fail(c, IMC_EINVAL, "imc_set_step_fields must be called before imc_clean");
   HIPCHK(hipEventRecord(c.ev0, c.stream), "event");
   HIPCHK(launch_clean(c, c.h_sc.n_total), "clean kernels");
+  // the census tally is computed by the clean kernels; bring it back in the same wait
+  c.h_rad.resize((size_t)c.C);
+  HIPCHK(hipMemcpyAsync(c.h_rad.data(), c.d_rad, 8 * (size_t)c.C, hipMemcpyDeviceToHost, c.stream), "rad D2H");
+  c.rad_valid = true;
   int rc = fetch_scalars(c);
   if (rc) return rc;
   HIPCHK(hipEventRecord(c.ev1, c.stream), "event");
@@ -407,6 +411,10 @@ int imc_tally_census(imc_ctx* h, double* radenergydens) {
   if (!h) return IMC_EINVAL;
   Ctx& c = h->c;
   if (!radenergydens) return fail(c, IMC_EINVAL, "radenergydens required");
+  if (c.rad_valid) {                              // copied back by imc_clean
+    std::memcpy(radenergydens, c.h_rad.data(), 8 * (size_t)c.C);
+    return IMC_OK;
+  }
   HIPCHK(hipEventRecord(c.ev0, c.stream), "event");
   HIPCHK(hipMemcpyAsync(radenergydens, c.d_rad, 8 * (size_t)c.C, hipMemcpyDeviceToHost, c.stream), "rad D2H");
   HIPCHK(hipEventRecord(c.ev1, c.stream), "event");
@@ -561,6 +569,7 @@ static int surr_alloc(Ctx& c) {
   ok &= dalloc(&c.d_surr_bias, 4 * (size_t)BIAS_TOTAL) == hipSuccess;
   ok &= dalloc(&c.d_surr_loss, 8) == hipSuccess;
   ok &= dalloc(&c.d_surr_events, 8) == hipSuccess;
+  ok &= dalloc(&c.d_surr_cells, 16 * (size_t)c.C) == hipSuccess;
   if (!ok) return fail(c, IMC_ENOMEM, "surrogate allocation failed");
   HIPCHK(launch_zero(c, c.d_surr_params, 4 * (size_t)MLP_NPARAM), "zero params");
   HIPCHK(launch_zero(c, c.d_surr_grad, 4 * (size_t)MLP_NPARAM), "zero grad");

@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 #include <string>
+#include <vector>
 
 #include "../../include/imc_hip.h"
 
@@ -136,7 +137,12 @@ struct Ctx {
   float* d_surr_bias = nullptr;      // FP32 biases
   float* d_surr_loss = nullptr;
   unsigned long long* d_surr_events = nullptr;
+  void* d_surr_cells = nullptr;      // [C] float4 per-cell decision inputs
   int surr_step = 0;
+
+  // host copies of the last transport tallies and census tally
+  std::vector<double> h_dep, h_rad;
+  bool rad_valid = false;
 
   // timing
   hipEvent_t ev0 = nullptr, ev1 = nullptr, evk0 = nullptr, evk1 = nullptr;

@@ -669,9 +669,24 @@ hipError_t launch_reduce_partials(Ctx& c, int nblocks, int tally_mode, int nbins
 struct SurrArgs {
   const uint16_t* wbf;
   const float* bias;
+  const float4* cells;   // per cell {tau_cell if inside the box else -1, sigma_c, dx s, f sigma_a}
   SurrBox box;
+  SurrMap map;
   int tally_bytes;
 };
+
+// Per-cell inputs of the surrogate decision, formed once per transport launch.
+template <int P>
+__global__ void k_surr_cells(int C, SurrBox bx, const typename Prec<P>::T* __restrict__ dxs,
+                             const typename Prec<P>::T* __restrict__ sigcol,
+                             const typename Prec<P>::W* __restrict__ sigdep, float4* __restrict__ out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= C) return;
+  const float scf = (float)sigcol[i], Lf = (float)dxs[i];
+  const float tau = scf * Lf;
+  const bool in = tau >= bx.tau_cell_min && tau <= bx.tau_cell_max;
+  out[i] = make_float4(in ? tau : -1.0f, scf, Lf, (float)sigdep[i]);
+}
 
 constexpr uint32_t TAG_SURR = 3u;
 constexpr int SURR_BATCH = 8;      // lanes that trigger one network evaluation
@@ -711,6 +726,7 @@ __global__ __launch_bounds__(TPB) void k_transport(SoA s, Geo g, Scalars* sc, vo
     for (int i = threadIdx.x; i < BF_TOTAL; i += blockDim.x) s_w[i] = sa.wbf[i];
     for (int i = threadIdx.x; i < BIAS_TOTAL; i += blockDim.x) s_b[i] = sa.bias[i];
   }
+  const SurrMap& smap = sa.map;
   const CellRec<P>* s_cell = nullptr;
   if (CELL) {
     CellRec<P>* w = (CellRec<P>*)(smem + g.cell_off);
@@ -727,7 +743,6 @@ __global__ __launch_bounds__(TPB) void k_transport(SoA s, Geo g, Scalars* sc, vo
 
   const T* __restrict__ dxs = (const T*)g.dxs;
   const T* __restrict__ dys = (const T*)g.dys;
-  const T* __restrict__ sigcol = (const T*)g.sigcol;
   const W* __restrict__ sigdep = (const W*)g.sigdep;
   const T* __restrict__ isig = (const T*)g.isig;
   const W* __restrict__ idxw = (const W*)g.idxw;
@@ -803,10 +818,10 @@ __global__ __launch_bounds__(TPB) void k_transport(SoA s, Geo g, Scalars* sc, vo
     float s_ST = 0.f, s_Scut = 0.f, s_SB = 0.f, s_sc = 0.f;
     if (SURR) {
       if (has) {
-        const float scf = (float)sigcol[ix], Lf = (float)dxs[ix];
-        const float tau_cell = scf * Lf;
-        if (tau_cell >= sa.box.tau_cell_min && tau_cell <= sa.box.tau_cell_max) {
-          const float sdf = (float)sigdep[ix];
+        const float4 cellv = sa.cells[ix];                   // {tau_cell or -1, sigma_c, dx s, f sigma_a}
+        const float tau_cell = cellv.x;
+        if (tau_cell > 0.f) {
+          const float scf = cellv.y, Lf = cellv.z, sdf = cellv.w;
           const float ST = (float)(cT * (dtT - t) * sT);
           const float Scut = sdf > 0.f ? __logf((float)E / (float)Emin) / sdf : INFINITY;
           const float SB = fminf(ST, Scut);
@@ -815,7 +830,7 @@ __global__ __launch_bounds__(TPB) void k_transport(SoA s, Geo g, Scalars* sc, vo
           if (depth >= sa.box.depth_min && surr_in_box(sa.box, tau_cell, tau_b, kappa)) {
             want = true;
             s_ST = ST; s_Scut = Scut; s_SB = SB; s_sc = scf;
-            surr_features(sa.box, tau_cell, (float)x / Lf, (float)mu, tau_b, kappa, feat);
+            surr_features(smap, tau_cell, (float)x / Lf, (float)mu, tau_b, kappa, feat);
           }
         }
       }
@@ -1185,7 +1200,19 @@ hipError_t launch_transport(Ctx& c, int grid, int tm, size_t lds) {
   sa.wbf = c.d_surr_wbf;
   sa.bias = c.d_surr_bias;
   sa.box = c.surr_box;
+  sa.map = surr_map(c.surr_box);
   sa.tally_bytes = (int)lds;
+  sa.cells = (const float4*)c.d_surr_cells;
+  if (use_surrogate(c)) {
+    const unsigned gcell = (unsigned)((c.C + 255) / 256);
+#define SC(P) k_surr_cells<P><<<gcell, 256, 0, c.stream>>>(c.C, c.surr_box, (const Prec<P>::T*)c.d_dxs,            \
+                                                           (const Prec<P>::T*)c.d_sigcol, (const Prec<P>::W*)c.d_sigdep, \
+                                                           (float4*)c.d_surr_cells)
+    if (c.prec == 64) SC(64);
+    else if (c.prec == 32) SC(32);
+    else SC(16);
+#undef SC
+  }
   const size_t total = lds + (use_surrogate(c) ? SURR_LDS : 0);
 #define TR(P, TM_, D, S, L) \
   k_transport<P, TM_, D, S, L><<<grid, TPB, total, c.stream>>>(s, g, c.d_sc, c.d_partial, c.d_gtally, sa)

@@ -12,7 +12,9 @@
 // Inference copy (BF16, transposed, zero padded): W1t[64][16], W2t[64][64], W3t[16][64]
 // plus FP32 biases b1[64], b2[64], b3[4].
 #pragma once
+#include <hip/hip_bf16.h>
 #include <hip/hip_runtime.h>
+#include <math.h>
 #include <stdint.h>
 
 #include "imc_internal.h"
@@ -30,11 +32,8 @@ constexpr int BIAS_TOTAL = 64 + 64 + 4;
 typedef short mshort4 __attribute__((ext_vector_type(4)));
 typedef float mfloat4 __attribute__((ext_vector_type(4)));
 
-// float -> bf16, round to nearest even (inputs are finite).
-__device__ __forceinline__ uint16_t f2bf(float f) {
-  const uint32_t u = __float_as_uint(f);
-  return (uint16_t)((u + 0x7FFFu + ((u >> 16) & 1u)) >> 16);
-}
+// float -> bf16, round to nearest even (gfx950's v_cvt_pk_bf16_f32; inputs are finite).
+__device__ __forceinline__ uint16_t f2bf(float f) { return __bfloat16_as_ushort(__float2bfloat16(f)); }
 __device__ __forceinline__ float bf2f(uint16_t h) { return __uint_as_float(((uint32_t)h) << 16); }
 
 __device__ __forceinline__ mfloat4 mfma_bf16(mshort4 a, mshort4 b, mfloat4 c) {
@@ -150,22 +149,40 @@ __device__ __forceinline__ void mlp_forward_wave(const uint16_t* __restrict__ wb
 // free path 1/sigma_c (sigma_c = sigma_a(1-f) + sigma_s):
 //   tau_cell = sigma_c * L, xi = x/L, mu, tau_B = sigma_c * min(census path, cut-off path),
 //   kappa = f sigma_a / sigma_c (weight decay per mean free path).
+// Logarithmic features are mapped to [-1, 1] over the training box: f = a ln(v) + b, with
+// (a, b) formed once per launch (SurrMap) instead of per sample.
 // ---------------------------------------------------------------------------------
-__device__ __forceinline__ float map_log(float v, float lo, float hi) {
-  return 2.0f * (__logf(v) - __logf(lo)) / (__logf(hi) - __logf(lo)) - 1.0f;
+struct SurrMap {
+  float a[4], b[4];                    // tau_cell, tau_B, kappa, tau_B / tau_cell^2
+};
+
+// Formed on the host and passed by value, so the coefficients live in scalar registers.
+inline void map_coef(float lo, float hi, float& a, float& b) {
+  const float l0 = logf(lo), l1 = logf(hi);
+  a = 2.0f / (l1 - l0);
+  b = -1.0f - a * l0;
 }
 
-__device__ __forceinline__ void surr_features(const SurrBox& bx, float tau_cell, float xi, float mu,
-                                              float tau_b, float kappa, float (&f)[MLP_IN]) {
-  f[0] = map_log(tau_cell, bx.tau_cell_min, bx.tau_cell_max);
+inline SurrMap surr_map(const SurrBox& bx) {
+  SurrMap m;
+  map_coef(bx.tau_cell_min, bx.tau_cell_max, m.a[0], m.b[0]);
+  map_coef(bx.tau_b_min, bx.tau_b_max, m.a[1], m.b[1]);
+  map_coef(bx.kappa_min, bx.kappa_max, m.a[2], m.b[2]);
+  // diffusion-time ratio tau_B / tau_cell^2 over its range in the box
+  map_coef(bx.tau_b_min / (bx.tau_cell_max * bx.tau_cell_max), bx.tau_b_max / (bx.tau_cell_min * bx.tau_cell_min),
+           m.a[3], m.b[3]);
+  return m;
+}
+
+__device__ __forceinline__ void surr_features(const SurrMap& m, float tau_cell, float xi, float mu, float tau_b,
+                                              float kappa, float (&f)[MLP_IN]) {
+  const float lc = __logf(tau_cell), lb = __logf(tau_b);
+  f[0] = m.a[0] * lc + m.b[0];
   f[1] = 2.0f * xi - 1.0f;
   f[2] = mu;
-  f[3] = map_log(tau_b, bx.tau_b_min, bx.tau_b_max);
-  f[4] = map_log(kappa, bx.kappa_min, bx.kappa_max);
-  // diffusion-time ratio tau_B / tau_cell^2 over its range in the box
-  const float lo = bx.tau_b_min / (bx.tau_cell_max * bx.tau_cell_max);
-  const float hi = bx.tau_b_max / (bx.tau_cell_min * bx.tau_cell_min);
-  f[5] = map_log(tau_b / (tau_cell * tau_cell), lo, hi);
+  f[3] = m.a[1] * lb + m.b[1];
+  f[4] = m.a[2] * __logf(kappa) + m.b[2];
+  f[5] = m.a[3] * (lb - 2.0f * lc) + m.b[3];
 }
 
 __device__ __forceinline__ bool surr_in_box(const SurrBox& bx, float tau_cell, float tau_b, float kappa) {

@@ -23,10 +23,11 @@ constexpr uint32_t GEN_KEY1 = 0x53555252u;
 constexpr int GEN_MAX_EVENTS = 1 << 20;
 
 // ---------------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_surr_generate(SurrBox bx, int64_t n, uint32_t seed,
+__global__ __launch_bounds__(256) void k_surr_generate(SurrBox bx, SurrMap smap, int64_t n, uint32_t seed,
                                                        float* __restrict__ feat, float* __restrict__ targ,
                                                        unsigned long long* __restrict__ events) {
   unsigned long long evcount = 0;
+
   const float lc0 = __logf(bx.tau_cell_min), lc1 = __logf(bx.tau_cell_max);
   const float lb0 = __logf(bx.tau_b_min), lb1 = __logf(bx.tau_b_max);
   const float lk0 = __logf(bx.kappa_min), lk1 = __logf(bx.kappa_max);
@@ -59,7 +60,7 @@ __global__ __launch_bounds__(256) void k_surr_generate(SurrBox bx, int64_t n, ui
       if (m == 0.0f) m = 0x1p-24f;
     }
     float f[MLP_IN];
-    surr_features(bx, tau_cell, xi, mu, tau_b, kappa, f);
+    surr_features(smap, tau_cell, xi, mu, tau_b, kappa, f);
 #pragma unroll
     for (int j = 0; j < MLP_IN; ++j) feat[i * MLP_IN + j] = f[j];
     targ[i * 4 + 0] = outcome == 0 ? 1.0f : 0.0f;
@@ -296,7 +297,8 @@ __global__ __launch_bounds__(64) void k_surr_forward(const uint16_t* __restrict_
 // ---------------------------------------------------------------------------------
 hipError_t surr_generate(Ctx& c, int64_t n, uint32_t seed, unsigned long long* d_events) {
   const int grid = (int)std::min<int64_t>((n + 255) / 256, 8L * c.num_cu * 4);
-  k_surr_generate<<<grid, 256, 0, c.stream>>>(c.surr_box, n, seed, c.d_surr_feat, c.d_surr_targ, d_events);
+  k_surr_generate<<<grid, 256, 0, c.stream>>>(c.surr_box, surr_map(c.surr_box), n, seed, c.d_surr_feat, c.d_surr_targ,
+                                              d_events);
   return hipGetLastError();
 }
 
