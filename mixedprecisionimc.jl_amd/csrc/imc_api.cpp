@@ -182,12 +182,15 @@ int imc_create(imc_ctx** out, const imc_config* cfg) {
   ok &= dalloc(&c.d_dys, T * ny) == hipSuccess;
   ok &= dalloc(&c.d_dxw, W * nx) == hipSuccess;
   ok &= dalloc(&c.d_dyw, W * ny) == hipSuccess;
-  ok &= dalloc(&c.d_temp, 8 * C) == hipSuccess;
-  ok &= dalloc(&c.d_fleck, 8 * C) == hipSuccess;
-  ok &= dalloc(&c.d_sa, 8 * C) == hipSuccess;
-  ok &= dalloc(&c.d_ss, 8 * C) == hipSuccess;
-  ok &= dalloc(&c.d_radsrc, 8 * C) == hipSuccess;
-  ok &= dalloc(&c.d_tsurf, 8 * (c.is2d ? 2 * nx + 2 * ny : 2)) == hipSuccess;
+  // step fields, one allocation in upload order: sigma_a, sigma_s, fleck, temp, radsource, t_surf
+  c.fields_len = 5 * C + (c.is2d ? 2 * nx + 2 * ny : 2);
+  ok &= dalloc(&c.d_fields, 8 * c.fields_len) == hipSuccess;
+  ok &= hipHostMalloc((void**)&c.h_fields, 8 * c.fields_len, hipHostMallocDefault) == hipSuccess;
+  ok &= hipEventCreateWithFlags(&c.ev_fields, hipEventDisableTiming) == hipSuccess;
+  if (ok) {
+    c.d_sa = c.d_fields; c.d_ss = c.d_sa + C; c.d_fleck = c.d_ss + C; c.d_temp = c.d_fleck + C;
+    c.d_radsrc = c.d_temp + C; c.d_tsurf = c.d_radsrc + C;
+  }
   ok &= dalloc(&c.d_sigcol, T * C) == hipSuccess;
   ok &= dalloc(&c.d_sigdep, W * C) == hipSuccess;
   ok &= dalloc(&c.d_isig, T * C) == hipSuccess;
@@ -231,13 +234,13 @@ int imc_destroy(imc_ctx* h) {
   Ctx& c = h->c;
   if (c.stream) (void)hipStreamSynchronize(c.stream);
   free_soa(c.buf[0]); free_soa(c.buf[1]);
-  void* ptrs[] = {c.d_dx, c.d_dy, c.d_dxs, c.d_dys, c.d_dxw, c.d_dyw, c.d_temp, c.d_fleck, c.d_sa, c.d_ss,
-                  c.d_radsrc, c.d_tsurf, c.d_sigcol, c.d_sigdep, c.d_isig, c.d_idxw, c.d_idyw, c.d_ent_e, c.d_ent_esc, c.d_ent_term,
+  void* ptrs[] = {c.d_dx, c.d_dy, c.d_dxs, c.d_dys, c.d_dxw, c.d_dyw, c.d_fields, c.d_sigcol, c.d_sigdep, c.d_isig, c.d_idxw, c.d_idyw, c.d_ent_e, c.d_ent_esc, c.d_ent_term,
                   c.d_ent_k, c.d_ent_cnt, c.d_ent_off, c.d_red, c.d_emitted, c.d_dep, c.d_gtally, c.d_rad, c.d_sc,
                   c.d_partial, c.d_tile, c.d_bhist, c.d_surr_feat, c.d_surr_targ, c.d_surr_params, c.d_surr_grad,
                   c.d_surr_m, c.d_surr_v, c.d_surr_wbf, c.d_surr_bias, c.d_surr_loss, c.d_surr_events, c.d_surr_cells};
   for (void* p : ptrs) (void)hipFree(p);
-  hipEvent_t evs[] = {c.ev0, c.ev1, c.evk0, c.evk1};
+  if (c.h_fields) (void)hipHostFree(c.h_fields);
+  hipEvent_t evs[] = {c.ev0, c.ev1, c.evk0, c.evk1, c.ev_fields};
   for (hipEvent_t e : evs)
     if (e) (void)hipEventDestroy(e);
   if (c.stream) (void)hipStreamDestroy(c.stream);
@@ -294,18 +297,23 @@ int imc_set_step_fields(imc_ctx* h, int64_t step, double dt, const double* sigma
   c.step = step;
   c.dt = dt;
   const size_t C = (size_t)c.C;
-  HIPCHK(hipMemcpyAsync(c.d_sa, sigma_a, 8 * C, hipMemcpyHostToDevice, c.stream), "sigma_a H2D");
-  HIPCHK(hipMemcpyAsync(c.d_ss, sigma_s, 8 * C, hipMemcpyHostToDevice, c.stream), "sigma_s H2D");
-  HIPCHK(hipMemcpyAsync(c.d_fleck, fleck, 8 * C, hipMemcpyHostToDevice, c.stream), "fleck H2D");
-  HIPCHK(hipMemcpyAsync(c.d_temp, temp, 8 * C, hipMemcpyHostToDevice, c.stream), "temp H2D");
-  HIPCHK(hipMemcpyAsync(c.d_radsrc, radsource, 8 * C, hipMemcpyHostToDevice, c.stream), "radsource H2D");
-  HIPCHK(hipMemcpyAsync(c.d_tsurf, ts.data(), 8 * ts.size(), hipMemcpyHostToDevice, c.stream), "tsurf H2D");
+  // The caller's arrays are copied into the pinned staging block before returning, and one
+  // asynchronous copy uploads it; the previous upload from that block is waited for first.
+  HIPCHK(hipEventSynchronize(c.ev_fields), "staging wait");
+  double* hf = c.h_fields;
+  const double* src[5] = {sigma_a, sigma_s, fleck, temp, radsource};
+  for (int q = 0; q < 5; ++q) std::memcpy(hf + q * C, src[q], 8 * C);
+  std::memcpy(hf + 5 * C, ts.data(), 8 * ts.size());
+  HIPCHK(hipMemcpyAsync(c.d_fields, hf, 8 * c.fields_len, hipMemcpyHostToDevice, c.stream), "step fields H2D");
+  HIPCHK(hipEventRecord(c.ev_fields, c.stream), "event");
   HIPCHK(launch_zero(c, (char*)c.d_sc + offsetof(Scalars, prep_underflow), sizeof(uint32_t)), "zero prep flag");
   HIPCHK(launch_prep(c), "prep kernel");
   uint32_t underflow = 0;
-  HIPCHK(hipMemcpyAsync(&underflow, (char*)c.d_sc + offsetof(Scalars, prep_underflow), sizeof(uint32_t),
-                        hipMemcpyDeviceToHost, c.stream), "prep flag D2H");
-  HIPCHK(hipStreamSynchronize(c.stream), "step fields sync");
+  if (c.prec == 16) {                   // only binary16 can round a mean free path to 0
+    HIPCHK(hipMemcpyAsync(&underflow, (char*)c.d_sc + offsetof(Scalars, prep_underflow), sizeof(uint32_t),
+                          hipMemcpyDeviceToHost, c.stream), "prep flag D2H");
+    HIPCHK(hipStreamSynchronize(c.stream), "step fields sync");
+  }
   if (underflow)
     return fail(c, IMC_ENUMERIC, (std::to_string(underflow) + " cell(s) have a collision mean free path "
                 "1/sigma_c that rounds to 0 in the tracking precision; use a wider PRECISION").c_str());

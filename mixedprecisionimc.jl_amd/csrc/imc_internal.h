@@ -89,8 +89,12 @@ struct Ctx {
   double* d_dx = nullptr; double* d_dy = nullptr;          // FP64 copies
   void* d_dxs = nullptr; void* d_dys = nullptr;            // T
   void* d_dxw = nullptr; void* d_dyw = nullptr;            // W
+  double* d_fields = nullptr;                              // the step fields below, one allocation
   double* d_temp = nullptr; double* d_fleck = nullptr; double* d_sa = nullptr; double* d_ss = nullptr;
   double* d_radsrc = nullptr; double* d_tsurf = nullptr;   // tsurf packed (1D: 2, 2D: 2nx+2ny)
+  double* h_fields = nullptr;                              // pinned staging block of the upload
+  size_t fields_len = 0;
+  hipEvent_t ev_fields = nullptr;                          // end of the last upload from h_fields
   void* d_sigcol = nullptr; void* d_sigdep = nullptr;
   void* d_isig = nullptr; void* d_idxw = nullptr; void* d_idyw = nullptr;   // reciprocals
 

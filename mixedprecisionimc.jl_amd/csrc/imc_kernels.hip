@@ -354,14 +354,24 @@ template <int P> struct Birth {
   uint32_t cell;
 };
 
-// Entry of birth b: last e with off[e] <= b.
-__device__ __forceinline__ int64_t birth_entry(const BirthArgs& A, int64_t b) {
-  int64_t lo = 0, hi = A.NE - 1;
+// Entry of birth b: last e with off[e] <= b (off in LDS when the entries fit, else HBM).
+__device__ __forceinline__ int64_t birth_entry(const int64_t* __restrict__ off, int64_t NE, int64_t b) {
+  int64_t lo = 0, hi = NE - 1;
   while (lo < hi) {
     const int64_t mid = (lo + hi + 1) >> 1;
-    if (A.ent_off[mid] <= b) lo = mid; else hi = mid - 1;
+    if (off[mid] <= b) lo = mid; else hi = mid - 1;
   }
   return lo;
+}
+
+constexpr int64_t ENT_LDS_MAX = 4096;    // entry offsets staged in LDS up to this many entries
+
+// Stages the entry offsets in LDS (when they fit) and returns the table to search.
+__device__ __forceinline__ const int64_t* stage_entries(const BirthArgs& A, int64_t* lds) {
+  if (A.NE > ENT_LDS_MAX) return A.ent_off;
+  for (int64_t i = threadIdx.x; i < A.NE; i += blockDim.x) lds[i] = A.ent_off[i];
+  __syncthreads();
+  return lds;
 }
 
 template <int P, bool TWO_D>
@@ -444,14 +454,16 @@ __device__ __forceinline__ int t_bucket(double t, double dt) {
 template <int P, bool TWO_D>
 __global__ __launch_bounds__(256) void k_birth_hist(BirthArgs A, int64_t ntiles, int64_t* __restrict__ hist) {
   __shared__ int cnt[BK];
+  extern __shared__ int64_t s_off[];
   if (threadIdx.x < BK) cnt[threadIdx.x] = 0;
+  const int64_t* off = stage_entries(A, s_off);
   __syncthreads();
   const int64_t tile = blockIdx.x;
 #pragma unroll 1
   for (int j = 0; j < BIRTH_ITEMS; ++j) {
     const int64_t b = tile * BIRTH_TILE + (int64_t)j * 256 + threadIdx.x;
     if (b < A.n_births) {
-      const Birth<P> o = birth_state<P, TWO_D>(A, b, birth_entry(A, b));
+      const Birth<P> o = birth_state<P, TWO_D>(A, b, birth_entry(off, A.NE, b));
       atomicAdd(&cnt[t_bucket(Tm<typename Prec<P>::T>::d(o.t), A.dt)], 1);
     }
   }
@@ -459,31 +471,29 @@ __global__ __launch_bounds__(256) void k_birth_hist(BirthArgs A, int64_t ntiles,
   if (threadIdx.x < BK) hist[(int64_t)threadIdx.x * ntiles + tile] = cnt[threadIdx.x];
 }
 
-// Single-block exclusive scan of n int64 values in place.
+// Single-block exclusive scan of n int64 values in place: each thread owns a contiguous
+// chunk (serial sum, block scan of the chunk sums, serial rewrite).
 __global__ __launch_bounds__(1024) void k_exscan_i64(int64_t* __restrict__ a, int64_t n) {
   __shared__ int64_t wtot[16];
-  __shared__ int64_t running;
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
-  if (tid == 0) running = 0;
-  __syncthreads();
-  for (int64_t base = 0; base < n; base += 1024) {
-    const int64_t i = base + tid;
-    const int64_t v = i < n ? a[i] : 0;
-    int64_t incl = v;
+  const int64_t per = (n + 1023) / 1024;
+  const int64_t lo = (int64_t)tid * per, hi = lo + per < n ? lo + per : n;
+  int64_t sum = 0;
+  for (int64_t i = lo; i < hi; ++i) sum += a[i];
+  int64_t incl = sum;
 #pragma unroll
-    for (int d = 1; d < 64; d <<= 1) {
-      const int64_t o = __shfl_up(incl, d, 64);
-      if (lane >= d) incl += o;
-    }
-    if (lane == 63) wtot[w] = incl;
-    __syncthreads();
-    int64_t wpre = 0;
-    for (int j = 0; j < w; ++j) wpre += wtot[j];
-    const int64_t r0 = running;
-    if (i < n) a[i] = r0 + wpre + incl - v;
-    __syncthreads();
-    if (tid == 1023) running = r0 + wpre + incl;
-    __syncthreads();
+  for (int d = 1; d < 64; d <<= 1) {
+    const int64_t o = __shfl_up(incl, d, 64);
+    if (lane >= d) incl += o;
+  }
+  if (lane == 63) wtot[w] = incl;
+  __syncthreads();
+  int64_t run = incl - sum;
+  for (int j = 0; j < w; ++j) run += wtot[j];
+  for (int64_t i = lo; i < hi; ++i) {
+    const int64_t v = a[i];
+    a[i] = run;
+    run += v;
   }
 }
 
@@ -498,6 +508,8 @@ __global__ __launch_bounds__(256) void k_births(BirthArgs A, int64_t ntiles, con
   using W = typename Prec<P>::W;
   __shared__ int run[BK];
   __shared__ int wc[4][BK];
+  extern __shared__ int64_t s_off[];
+  const int64_t* eoff = stage_entries(A, s_off);
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   const uint64_t lt_mask = lane == 0 ? 0ull : (~0ull >> (64 - lane));
   if (tid < BK) run[tid] = 0;
@@ -513,7 +525,7 @@ __global__ __launch_bounds__(256) void k_births(BirthArgs A, int64_t ntiles, con
     int64_t e = 0;
     int bk = 0;
     if (valid) {
-      e = birth_entry(A, b);
+      e = birth_entry(eoff, A.NE, b);
       o = birth_state<P, TWO_D>(A, b, e);
       bk = t_bucket(Tm<T>::d(o.t), A.dt);
     }
@@ -559,11 +571,12 @@ hipError_t launch_births(Ctx& c, int64_t n_births) {
   const int64_t ntiles = (n_births + BIRTH_TILE - 1) / BIRTH_TILE;
   if ((int64_t)BK * ntiles > c.bhist_len) return hipErrorInvalidValue;   // sized by the caller
   const unsigned grid = (unsigned)ntiles;
+  const size_t elds = c.NE <= ENT_LDS_MAX ? sizeof(int64_t) * (size_t)c.NE : 0;
 #define BIR(P, D)                                                                               \
   do {                                                                                          \
-    k_birth_hist<P, D><<<grid, 256, 0, c.stream>>>(A, ntiles, c.d_bhist);                       \
+    k_birth_hist<P, D><<<grid, 256, elds, c.stream>>>(A, ntiles, c.d_bhist);                    \
     k_exscan_i64<<<1, 1024, 0, c.stream>>>(c.d_bhist, (int64_t)BK * ntiles);                    \
-    k_births<P, D><<<grid, 256, 0, c.stream>>>(A, ntiles, c.d_bhist);                           \
+    k_births<P, D><<<grid, 256, elds, c.stream>>>(A, ntiles, c.d_bhist);                        \
   } while (0)
   if (c.prec == 64) { if (c.is2d) BIR(64, true); else BIR(64, false); }
   else if (c.prec == 32) { if (c.is2d) BIR(32, true); else BIR(32, false); }
