@@ -331,29 +331,47 @@ int imc_source(imc_ctx* h, int64_t n_input, int64_t n_max, double cellmin, doubl
   HIPCHK(launch_source_entries(c), "source entries kernel");
   HIPCHK(launch_tree_sum(c, c.d_ent_term, c.NE, (double*)((char*)c.d_sc + offsetof(Scalars, E_tot))),
          "energy sum kernel");
-  HIPCHK(launch_counts_and_scan(c, n_input, n_max, cellmin), "count kernels");
-  int rc = fetch_scalars(c);
-  if (rc) return rc;
-  if (!std::isfinite(c.h_sc.E_tot) || !(c.h_sc.E_tot > 0.0))
-    return fail(c, IMC_ENUMERIC, "total source energy is zero or not finite");
-  const int64_t need = c.h_sc.n_census + c.h_sc.n_births;
-  rc = grow(c, need, c.h_sc.n_census);
-  if (rc) return rc;
-  const int64_t hl = birth_hist_len(c.h_sc.n_births);
-  if (hl > c.bhist_len) {                          // per-tile bucket counts of the birth order
-    (void)hipFree(c.d_bhist);
-    c.d_bhist = nullptr;
-    c.bhist_len = 0;
-    if (dalloc(&c.d_bhist, sizeof(int64_t) * (size_t)(hl + hl / 4)) != hipSuccess)
-      return fail(c, IMC_ENOMEM, "birth-order scratch allocation failed");
-    c.bhist_len = hl + hl / 4;
+  // Births per entry are max(round(share), cellmin) with shares summing to the budget
+  // (imc_sourcing.jl:139-156), so budget + NE (cellmin + 1) bounds them (twice the budget
+  // with quirk Q1).  When that bound fits the buffers, the births are launched over it
+  // without a host round trip; the count kernel still refuses births that do not fit.
+  const int64_t n_census = c.h_sc.n_census;
+  const double budget = std::ceil(std::max((double)n_input, cellmin));
+  const double bound_d = budget * ((c.cfg.quirks & IMC_QUIRK_Q1_RADSRC_NBODY) ? 2.0 : 1.0) +
+                         (double)c.NE * (std::ceil(cellmin) + 1.0) + 16.0;
+  const int64_t bound = bound_d < 4e18 ? (int64_t)bound_d : INT64_MAX / 4;
+  const bool direct = n_census + bound <= c.buf[c.cur].cap && birth_hist_len(bound) <= c.bhist_len;
+  HIPCHK(launch_counts_and_scan(c, n_input, n_max, cellmin, direct ? c.buf[c.cur].cap : INT64_MAX), "count kernels");
+  int rc;
+  if (direct) {
+    HIPCHK(launch_births(c, bound), "birth kernel");
+  } else {
+    rc = fetch_scalars(c);
+    if (rc) return rc;
+    if (!std::isfinite(c.h_sc.E_tot) || !(c.h_sc.E_tot > 0.0))
+      return fail(c, IMC_ENUMERIC, "total source energy is zero or not finite");
+    rc = grow(c, c.h_sc.n_census + c.h_sc.n_births, c.h_sc.n_census);
+    if (rc) return rc;
+    const int64_t hl = birth_hist_len(c.h_sc.n_births);
+    if (hl > c.bhist_len) {                          // per-tile bucket counts of the birth order
+      (void)hipFree(c.d_bhist);
+      c.d_bhist = nullptr;
+      c.bhist_len = 0;
+      if (dalloc(&c.d_bhist, sizeof(int64_t) * (size_t)(hl + hl / 4)) != hipSuccess)
+        return fail(c, IMC_ENOMEM, "birth-order scratch allocation failed");
+      c.bhist_len = hl + hl / 4;
+    }
+    HIPCHK(launch_births(c, c.h_sc.n_births), "birth kernel");
   }
-  HIPCHK(launch_births(c, c.h_sc.n_births), "birth kernel");
   if (emittedenergy)
     HIPCHK(hipMemcpyAsync(emittedenergy, c.d_emitted, 8 * (size_t)c.C * c.K, hipMemcpyDeviceToHost, c.stream), "emitted D2H");
+  HIPCHK(hipMemcpyAsync(&c.h_sc, c.d_sc, sizeof(Scalars), hipMemcpyDeviceToHost, c.stream), "scalars D2H");
   HIPCHK(hipEventRecord(c.ev1, c.stream), "event");
   HIPCHK(hipStreamSynchronize(c.stream), "source sync");
   c.timing.ms_source += elapsed(c.ev0, c.ev1);
+  if (!std::isfinite(c.h_sc.E_tot) || !(c.h_sc.E_tot > 0.0))
+    return fail(c, IMC_ENUMERIC, "total source energy is zero or not finite");
+  if (c.h_sc.birth_overflow) return fail(c, IMC_ENOMEM, "births exceed the particle buffer");
   if (totalenergy) *totalenergy = c.h_sc.E_tot;
   if (n_alive) *n_alive = c.h_sc.n_total;
   return IMC_OK;

@@ -45,7 +45,7 @@ struct Scalars {
   uint32_t work;               // transport work-queue cursor
   uint32_t truncated;          // histories ended by the per-history event cap
   uint32_t prep_underflow;     // cells whose 1/sigma_c rounds to 0 in the tracking type
-  uint32_t pad2;
+  uint32_t birth_overflow;     // the step's births did not fit the particle buffer
 };
 
 struct SurrBox {               // training box of the thick-cell surrogate (see imc_mlp.h)
@@ -160,7 +160,7 @@ int state_bytes(int prec, bool is2d);
 hipError_t launch_prep(Ctx& c);
 hipError_t launch_source_entries(Ctx& c);
 hipError_t launch_tree_sum(Ctx& c, const double* in, int64_t n, double* out_scalar);
-hipError_t launch_counts_and_scan(Ctx& c, int64_t n_input, int64_t n_max, double cellmin);
+hipError_t launch_counts_and_scan(Ctx& c, int64_t n_input, int64_t n_max, double cellmin, int64_t cap);
 hipError_t launch_births(Ctx& c, int64_t n_births);
 int64_t birth_hist_len(int64_t n_births);
 hipError_t launch_transport(Ctx& c, int grid_blocks, int tally_mode, size_t lds_bytes);

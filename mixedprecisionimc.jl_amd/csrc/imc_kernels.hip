@@ -269,12 +269,13 @@ __global__ void k_counts(int nx, int ny, int C, int64_t NE, int quirks, int64_t 
   } else {
     cnt = en > 0 ? fmax(rint(x), cellmin) : 0.0;
   }
+  cnt = fmin(cnt, 1e12);           // a non-finite total energy must not overflow the sums
   const int64_t n_div = cnt > 0 ? (int64_t)cnt : 0;
   int64_t n_loop = n_div;
   if (TWO_D && (quirks & IMC_QUIRK_Q1_RADSRC_NBODY) && e >= nf + C && n_div > 0) {
     const int64_t eb = e - C;                                // body entry of the same cell
     const double xb = ent_term[eb] * n_source / E_tot;
-    const double cb = fmax(rint(xb), cellmin);
+    const double cb = fmin(fmax(rint(xb), cellmin), 1e12);
     n_loop = cb > 0 ? (int64_t)cb : 0;
   }
   ent_cnt[e] = n_loop;
@@ -283,7 +284,7 @@ __global__ void k_counts(int nx, int ny, int C, int64_t NE, int quirks, int64_t 
 
 // Single-block exclusive scan of the birth counts; sets n_births / n_total.
 __global__ __launch_bounds__(1024) void k_scan_counts(const int64_t* __restrict__ cnt, int64_t n,
-                                                      int64_t* __restrict__ off, Scalars* sc) {
+                                                      int64_t* __restrict__ off, Scalars* sc, int64_t cap) {
   __shared__ int64_t wtot[16];
   __shared__ int64_t running;
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
@@ -309,12 +310,19 @@ __global__ __launch_bounds__(1024) void k_scan_counts(const int64_t* __restrict_
     __syncthreads();
   }
   if (tid == 0) {
-    sc->n_births = running;
-    sc->n_total = sc->n_census + running;
+    if (running < 0 || running > cap - sc->n_census) {   // more births than slots: none, flagged
+      sc->n_births = 0;
+      sc->n_total = sc->n_census;
+      sc->birth_overflow = 1u;
+    } else {
+      sc->n_births = running;
+      sc->n_total = sc->n_census + running;
+      sc->birth_overflow = 0u;
+    }
   }
 }
 
-hipError_t launch_counts_and_scan(Ctx& c, int64_t n_input, int64_t n_max, double cellmin) {
+hipError_t launch_counts_and_scan(Ctx& c, int64_t n_input, int64_t n_max, double cellmin, int64_t cap) {
   const unsigned grid = (unsigned)((c.NE + 255) / 256);
 #define CNT(P, D)                                                                               \
   k_counts<P, D><<<grid, 256, 0, c.stream>>>(c.nx, c.ny, c.C, c.NE, c.cfg.quirks, n_input,     \
@@ -326,7 +334,7 @@ hipError_t launch_counts_and_scan(Ctx& c, int64_t n_input, int64_t n_max, double
 #undef CNT
   hipError_t err = hipGetLastError();
   if (err != hipSuccess) return err;
-  k_scan_counts<<<1, 1024, 0, c.stream>>>(c.d_ent_cnt, c.NE, c.d_ent_off, c.d_sc);
+  k_scan_counts<<<1, 1024, 0, c.stream>>>(c.d_ent_cnt, c.NE, c.d_ent_off, c.d_sc, cap);
   return hipGetLastError();
 }
 
@@ -459,10 +467,11 @@ __global__ __launch_bounds__(256) void k_birth_hist(BirthArgs A, int64_t ntiles,
   const int64_t* off = stage_entries(A, s_off);
   __syncthreads();
   const int64_t tile = blockIdx.x;
+  const int64_t nbirths = A.sc->n_births;
 #pragma unroll 1
   for (int j = 0; j < BIRTH_ITEMS; ++j) {
     const int64_t b = tile * BIRTH_TILE + (int64_t)j * 256 + threadIdx.x;
-    if (b < A.n_births) {
+    if (b < nbirths) {
       const Birth<P> o = birth_state<P, TWO_D>(A, b, birth_entry(off, A.NE, b));
       atomicAdd(&cnt[t_bucket(Tm<typename Prec<P>::T>::d(o.t), A.dt)], 1);
     }
@@ -515,12 +524,13 @@ __global__ __launch_bounds__(256) void k_births(BirthArgs A, int64_t ntiles, con
   if (tid < BK) run[tid] = 0;
   const int64_t tile = blockIdx.x;
   const int64_t n_census = A.sc->n_census;
+  const int64_t nbirths = A.sc->n_births;
   T* X = (T*)A.out.x; T* MU = (T*)A.out.mu; T* TT = (T*)A.out.t;
   W* EE = (W*)A.out.E; W* E0 = (W*)A.out.E0;
 #pragma unroll 1
   for (int j = 0; j < BIRTH_ITEMS; ++j) {
     const int64_t b = tile * BIRTH_TILE + (int64_t)j * 256 + tid;
-    const bool valid = b < A.n_births;
+    const bool valid = b < nbirths;
     Birth<P> o;
     int64_t e = 0;
     int bk = 0;
@@ -558,6 +568,8 @@ __global__ __launch_bounds__(256) void k_births(BirthArgs A, int64_t ntiles, con
   }
 }
 
+// n_births: the step's birth count or an upper bound of it (the kernels read the count from
+// the device scalars; tiles past it do nothing).
 hipError_t launch_births(Ctx& c, int64_t n_births) {
   if (n_births <= 0) return hipSuccess;
   BirthArgs A;

@@ -188,3 +188,31 @@ def test_fp16_rejects_a_mean_free_path_it_cannot_represent():
             with pytest.raises(ImcError, match="mean free path"):
                 engine_step(e, case, 0)
         e.close()
+
+
+@pytest.mark.parametrize("prec", [64, 32])
+@pytest.mark.parametrize("geom", [1, 2])
+def test_second_step_births_match_oracle(prec, geom):
+    """The first sourcing sizes the buffers; later ones launch the births over an upper bound
+    of their count without a host round trip.  The second step's population (census
+    survivors of step 0, then the step-1 births) equals the oracle's sourcing from that
+    census, bit for bit."""
+    case = case1d(prec=prec) if geom == 1 else case2d(prec=prec)
+    print(f"[second step births] geometry={geom} prec={prec}", flush=True)
+    e = engine(case)
+    engine_step(e, case, 0)
+    e.source(3000, 30000, 2.0)
+    e.transport()
+    e.clean()
+    census = e.export_particles()
+    engine_step(e, case, 1)
+    emitted, etot, n = e.source(3000, 30000, 2.0)
+    g = e.export_particles()
+    op, st = oracle_problem(case)
+    rec, o_emit, o_etot, nb = op.source(st, 1, census, 3000, 30000, 2.0)
+    assert n == rec.shape[0] == g.shape[0], (n, rec.shape, g.shape)
+    assert etot == pytest.approx(o_etot, rel=1e-14)
+    np.testing.assert_array_equal(_cells(g, geom), _cells(rec, geom))
+    for name, idx in _fields(geom):
+        np.testing.assert_array_equal(g[:, idx], rec[:, idx], err_msg=name)
+    e.close()
