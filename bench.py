@@ -30,7 +30,7 @@ DECK_SLAB = os.path.join(ROOT, "decks", "c1_slab_fp64.txt")
 HBM_PEAK_GBS = 8000.0
 
 
-TALLY_MODES = {0: "WAVEROW", 1: "BLOCKROW", 2: "GLOBAL"}
+TALLY_MODES = {0: "WAVEROW", 1: "BLOCKROW", 2: "GLOBAL", 3: "HASH"}
 
 
 def profiled_traffic():

@@ -51,7 +51,7 @@ typedef struct {
   double phys_c, phys_a, distancescale;  /* constants.jl:9-12, imc_mesh.jl:131 */
   int64_t capacity;          /* initial particle capacity; grows on demand */
   /* Tuning, 0 = automatic.  reserved[0]: transport tally mode (1 per-wave LDS rows,
-     2 per-block LDS row, 3 global FP64 atomics).  reserved[1]: transport blocks per CU
+     2 per-block LDS row, 3 global FP64 atomics, 4 hashed per-block LDS table).  reserved[1]: transport blocks per CU
      (upper bound on the occupancy-derived persistent grid; 0 means 5).  Others must be 0. */
   int32_t reserved[8];
 } imc_config;
@@ -64,7 +64,7 @@ typedef struct {
   int64_t histories;              /* particles handed to transport */
   int64_t census_out;             /* particles written back to census */
   int64_t segments;               /* event-loop passes (the reference's `iterations`) */
-  int32_t transport_tally_mode;   /* 0 wave rows, 1 block row (LDS atomics), 2 global atomics */
+  int32_t transport_tally_mode;   /* 0 wave rows, 1 block row (LDS atomics), 2 global atomics, 3 hashed */
   int32_t transport_grid;         /* persistent transport blocks of the last launch */
   int64_t surrogate_events;       /* thick-cell walks replaced by the surrogate */
   int64_t truncated;              /* histories ended by the 2^24-event cap (non-finite state) */

@@ -185,7 +185,7 @@ class Engine:
         cfg.distancescale = distancescale
         cfg.capacity = int(capacity)
         cfg.quirks = quirks
-        cfg.reserved[0] = int(tally_mode)        # 0 auto, 1 wave rows, 2 block row, 3 global atomics
+        cfg.reserved[0] = int(tally_mode)        # 0 auto, 1 wave rows, 2 block row, 3 global, 4 hashed
         cfg.reserved[1] = int(blocks_per_cu)     # 0 auto: occupancy-derived persistent grid
         self.geometry = geometry
         self.precision = precision

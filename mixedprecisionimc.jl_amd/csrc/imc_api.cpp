@@ -386,11 +386,11 @@ int imc_transport(imc_ctx* h, double* energydep, double* lostenergy, int64_t* se
   HIPCHK(hipEventRecord(c.ev0, c.stream), "event");
   HIPCHK(launch_zero(c, (char*)c.d_sc + offsetof(Scalars, segments), sizeof(Scalars) - offsetof(Scalars, segments)),
          "zero scalars");
-  if (tm == TM_GLOBAL) HIPCHK(launch_zero(c, c.d_gtally, 8 * (size_t)c.NB * gtally_copies(c.NB)), "zero tallies");
+  if (tm == TM_GLOBAL || tm == TM_HASH) HIPCHK(launch_zero(c, c.d_gtally, 8 * (size_t)c.NB * gtally_copies(c.NB)), "zero tallies");
   HIPCHK(hipEventRecord(c.evk0, c.stream), "event");
   HIPCHK(launch_transport(c, grid, tm, lds), "transport kernel");
   HIPCHK(hipEventRecord(c.evk1, c.stream), "event");
-  if (tm == TM_GLOBAL) HIPCHK(launch_sum_copies(c, c.d_gtally, gtally_copies(c.NB), c.NB, c.d_dep), "tally copies");
+  if (tm == TM_GLOBAL || tm == TM_HASH) HIPCHK(launch_sum_copies(c, c.d_gtally, gtally_copies(c.NB), c.NB, c.d_dep), "tally copies");
   else HIPCHK(launch_reduce_partials(c, grid, tm, c.NB, c.d_dep), "tally reduction");
   c.h_dep.resize((size_t)c.NB);                 // C*K bins + lost energy, one copy
   HIPCHK(hipMemcpyAsync(c.h_dep.data(), c.d_dep, 8 * (size_t)c.NB, hipMemcpyDeviceToHost, c.stream), "energydep D2H");

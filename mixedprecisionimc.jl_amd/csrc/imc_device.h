@@ -229,7 +229,17 @@ __device__ __forceinline__ void neumaier(double& s, double& c, double v) {
 //            applies a Neumaier update -- no atomics, compensated, deterministic.
 //  GLOBAL  : lanes add straight into the global FP64 bins with atomics (meshes whose
 //            bins do not fit in LDS).
-enum TallyMode { TM_WAVEROW = 0, TM_BLOCKROW = 1, TM_GLOBAL = 2 };
+enum TallyMode { TM_WAVEROW = 0, TM_BLOCKROW = 1, TM_GLOBAL = 2, TM_HASH = 3 };
+
+// TM_HASH: an open LDS table of HASH_SLOTS (bin, value) slots per block for meshes whose bin
+// row does not fit in LDS.  A bin claims its slot (multiplicative hash) on first touch;
+// deposits to claimed bins are LDS atomics, a bin whose slot holds another bin goes to the
+// global copy, and the block adds every claimed slot to the global copy once at the end.
+constexpr int HASH_BITS = 12;
+constexpr int HASH_SLOTS = 1 << HASH_BITS;
+__device__ __forceinline__ uint32_t hash_slot(int bin) {
+  return ((uint32_t)bin * 2654435761u) >> (32 - HASH_BITS);
+}
 
 template <class Acc, int TM>
 struct Tally {
@@ -245,6 +255,19 @@ struct Tally {
     }
     if (TM == TM_GLOBAL) {
       if (bin >= 0) atomicAdd(&gbins[bin], (double)val);
+      return;
+    }
+    if (TM == TM_HASH) {
+      if (bin < 0) return;
+      int* tags = (int*)(lds + HASH_SLOTS);
+      const uint32_t s = hash_slot(bin);
+      int t = tags[s];
+      if (t == -1) {
+        t = atomicCAS(&tags[s], -1, bin);
+        if (t == -1) t = bin;
+      }
+      if (t == bin) atomicAdd(&lds[s], val);
+      else atomicAdd(&gbins[bin], (double)val);
       return;
     }
     const int lane = lane_id();

@@ -606,7 +606,10 @@ int64_t birth_hist_len(int64_t n_births) {
 // ---------------------------------------------------------------------------------
 template <class Acc, int TM>
 __device__ __forceinline__ void tally_init(Acc* lds, int NB) {
-  if (TM == TM_WAVEROW) {
+  if (TM == TM_HASH) {
+    int* tags = (int*)(lds + HASH_SLOTS);
+    for (int i = threadIdx.x; i < HASH_SLOTS; i += blockDim.x) { lds[i] = (Acc)0; tags[i] = -1; }
+  } else if (TM == TM_WAVEROW) {
     for (int i = threadIdx.x; i < NWAVES * NB * 2; i += blockDim.x) lds[i] = (Acc)0;
   } else if (TM == TM_BLOCKROW) {
     for (int i = threadIdx.x; i < NB; i += blockDim.x) lds[i] = (Acc)0;
@@ -617,7 +620,7 @@ __device__ __forceinline__ void tally_init(Acc* lds, int NB) {
 // Writes this block's (sum, compensation) per bin to partial[blockIdx.x][bin][2].
 template <class Acc, int TM>
 __device__ __forceinline__ void tally_flush(const Acc* lds, int NB, Acc* partial) {
-  if (TM == TM_GLOBAL) return;
+  if (TM == TM_GLOBAL || TM == TM_HASH) return;
   __syncthreads();
   Acc* out = partial + (size_t)blockIdx.x * NB * 2;
   for (int j = threadIdx.x; j < NB; j += blockDim.x) {
@@ -764,7 +767,8 @@ __global__ __launch_bounds__(TPB) void k_transport(SoA s, Geo g, Scalars* sc, vo
     s_cell = w;
   }
   tally_init<Acc, TM>(lds, NB);       // its barrier also publishes the cell table
-  Tally<Acc, TM> tal{lds, TM == TM_GLOBAL ? gbins + (size_t)(blockIdx.x % g.gcopies) * NB : gbins, NB};
+  Tally<Acc, TM> tal{lds, (TM == TM_GLOBAL || TM == TM_HASH) ? gbins + (size_t)(blockIdx.x % g.gcopies) * NB : gbins,
+                     NB};
 
   const T* __restrict__ dxs = (const T*)g.dxs;
   const T* __restrict__ dys = (const T*)g.dys;
@@ -1104,21 +1108,30 @@ __global__ __launch_bounds__(TPB) void k_transport(SoA s, Geo g, Scalars* sc, vo
     if (ntrunc) atomicAdd(&sc->truncated, ntrunc);
     if (SURR) atomicAdd(&sc->surr_events, nsurr);
   }
+  if (TM == TM_HASH) {                    // claimed slots go to the block's global copy once
+    __syncthreads();
+    const int* tags = (const int*)(lds + HASH_SLOTS);
+    for (int j = threadIdx.x; j < HASH_SLOTS; j += blockDim.x)
+      if (tags[j] >= 0) atomicAdd(&tal.gbins[tags[j]], (double)lds[j]);
+  }
   tally_flush<Acc, TM>(lds, NB, (Acc*)partial_v);
 }
 
-// Tally mode: cfg.reserved[0] (0 auto, 1 WAVEROW, 2 BLOCKROW, 3 GLOBAL).  Auto keeps a
-// block row in LDS when it fits in 64 KiB and otherwise adds into global bins.
+// Tally mode: cfg.reserved[0] (0 auto, 1 WAVEROW, 2 BLOCKROW, 3 GLOBAL, 4 HASH).  Auto keeps
+// a block row in LDS when it fits in 64 KiB and otherwise uses the hashed LDS table, whose
+// claimed slots flush once per block and whose colliding bins add into global copies.
 static void pick_tally(const Ctx& c, int nbins, int* tally_mode, size_t* lds_bytes) {
   const size_t acc = c.prec == 64 ? 8 : 4;
   const size_t wave_rows = (size_t)NWAVES * nbins * 2 * acc, block_row = (size_t)nbins * acc;
   int req = c.cfg.reserved[0];
   if (req == 1 && wave_rows > 64 * 1024) req = 0;
   if (req == 2 && block_row > 64 * 1024) req = 0;
+  const size_t hash_bytes = (size_t)HASH_SLOTS * (acc + 4);
   if (req == 1) { *tally_mode = TM_WAVEROW; *lds_bytes = wave_rows; }
   else if (req == 3) { *tally_mode = TM_GLOBAL; *lds_bytes = 0; }
+  else if (req == 4) { *tally_mode = TM_HASH; *lds_bytes = hash_bytes; }
   else if (block_row <= 64 * 1024) { *tally_mode = TM_BLOCKROW; *lds_bytes = block_row; }
-  else { *tally_mode = TM_GLOBAL; *lds_bytes = 0; }
+  else { *tally_mode = TM_HASH; *lds_bytes = hash_bytes; }
   *lds_bytes = (*lds_bytes + 15) & ~(size_t)15;
 }
 
@@ -1133,72 +1146,43 @@ static size_t cell_table_bytes(const Ctx& c) {
 }
 
 // Dispatch over (precision, tally mode, geometry, surrogate, LDS cell table).  F(P, TM, D, S, L)
-// expands to a statement using the kernel k_transport<P, TM, D, S, L>.
+// expands to a statement using the kernel k_transport<P, TM, D, S, L>.  The surrogate variants
+// exist for the block-row and global modes only.
+#define IMC_TM_DISPATCH(F, P, D, S, L)                                                   \
+  do {                                                                                   \
+    if (tm == TM_WAVEROW) F(P, TM_WAVEROW, D, S, L);                                     \
+    else if (tm == TM_BLOCKROW) F(P, TM_BLOCKROW, D, S, L);                              \
+    else if (tm == TM_HASH) F(P, TM_HASH, D, S, L);                                      \
+    else F(P, TM_GLOBAL, D, S, L);                                                       \
+  } while (0)
+#define IMC_TM_DISPATCH_SURR(F, P, L)                                                    \
+  do {                                                                                   \
+    if (tm == TM_BLOCKROW) F(P, TM_BLOCKROW, false, true, L);                            \
+    else F(P, TM_GLOBAL, false, true, L);                                                \
+  } while (0)
+#define IMC_TRANSPORT_DISPATCH_P(F, P)                                                   \
+  do {                                                                                   \
+    if (c.is2d) IMC_TM_DISPATCH(F, P, true, false, false);                               \
+    else if (cell_) { if (surr_) IMC_TM_DISPATCH_SURR(F, P, true);                       \
+                      else IMC_TM_DISPATCH(F, P, false, false, true); }                  \
+    else { if (surr_) IMC_TM_DISPATCH_SURR(F, P, false);                                 \
+           else IMC_TM_DISPATCH(F, P, false, false, false); }                            \
+  } while (0)
 #define IMC_TRANSPORT_DISPATCH(F)                                                        \
   do {                                                                                   \
     const bool surr_ = use_surrogate(c);                                                 \
     const bool cell_ = cell_table_bytes(c) > 0;                                          \
-    if (c.prec == 64) {                                                                  \
-      if (c.is2d) { if (tm == TM_WAVEROW) F(64, TM_WAVEROW, true, false, false);         \
-                    else if (tm == TM_BLOCKROW) F(64, TM_BLOCKROW, true, false, false);  \
-                    else F(64, TM_GLOBAL, true, false, false); }                         \
-      else if (cell_) {                                                                  \
-        if (surr_) { if (tm == TM_GLOBAL) F(64, TM_GLOBAL, false, true, true);           \
-                     else F(64, TM_BLOCKROW, false, true, true); }                       \
-        else if (tm == TM_WAVEROW) F(64, TM_WAVEROW, false, false, true);                \
-        else if (tm == TM_BLOCKROW) F(64, TM_BLOCKROW, false, false, true);              \
-        else F(64, TM_GLOBAL, false, false, true);                                       \
-      }                                                                                  \
-      else {                                                                             \
-        if (surr_) { if (tm == TM_GLOBAL) F(64, TM_GLOBAL, false, true, false);          \
-                     else F(64, TM_BLOCKROW, false, true, false); }                      \
-        else if (tm == TM_WAVEROW) F(64, TM_WAVEROW, false, false, false);               \
-        else if (tm == TM_BLOCKROW) F(64, TM_BLOCKROW, false, false, false);             \
-        else F(64, TM_GLOBAL, false, false, false);                                      \
-      }                                                                                  \
-    } else if (c.prec == 32) {                                                           \
-      if (c.is2d) { if (tm == TM_WAVEROW) F(32, TM_WAVEROW, true, false, false);         \
-                    else if (tm == TM_BLOCKROW) F(32, TM_BLOCKROW, true, false, false);  \
-                    else F(32, TM_GLOBAL, true, false, false); }                         \
-      else if (cell_) {                                                                  \
-        if (surr_) { if (tm == TM_GLOBAL) F(32, TM_GLOBAL, false, true, true);           \
-                     else F(32, TM_BLOCKROW, false, true, true); }                       \
-        else if (tm == TM_WAVEROW) F(32, TM_WAVEROW, false, false, true);                \
-        else if (tm == TM_BLOCKROW) F(32, TM_BLOCKROW, false, false, true);              \
-        else F(32, TM_GLOBAL, false, false, true);                                       \
-      }                                                                                  \
-      else {                                                                             \
-        if (surr_) { if (tm == TM_GLOBAL) F(32, TM_GLOBAL, false, true, false);          \
-                     else F(32, TM_BLOCKROW, false, true, false); }                      \
-        else if (tm == TM_WAVEROW) F(32, TM_WAVEROW, false, false, false);               \
-        else if (tm == TM_BLOCKROW) F(32, TM_BLOCKROW, false, false, false);             \
-        else F(32, TM_GLOBAL, false, false, false);                                      \
-      }                                                                                  \
-    } else {                                                                             \
-      if (c.is2d) { if (tm == TM_WAVEROW) F(16, TM_WAVEROW, true, false, false);         \
-                    else if (tm == TM_BLOCKROW) F(16, TM_BLOCKROW, true, false, false);  \
-                    else F(16, TM_GLOBAL, true, false, false); }                         \
-      else if (cell_) {                                                                  \
-        if (surr_) { if (tm == TM_GLOBAL) F(16, TM_GLOBAL, false, true, true);           \
-                     else F(16, TM_BLOCKROW, false, true, true); }                       \
-        else if (tm == TM_WAVEROW) F(16, TM_WAVEROW, false, false, true);                \
-        else if (tm == TM_BLOCKROW) F(16, TM_BLOCKROW, false, false, true);              \
-        else F(16, TM_GLOBAL, false, false, true);                                       \
-      }                                                                                  \
-      else {                                                                             \
-        if (surr_) { if (tm == TM_GLOBAL) F(16, TM_GLOBAL, false, true, false);          \
-                     else F(16, TM_BLOCKROW, false, true, false); }                      \
-        else if (tm == TM_WAVEROW) F(16, TM_WAVEROW, false, false, false);               \
-        else if (tm == TM_BLOCKROW) F(16, TM_BLOCKROW, false, false, false);             \
-        else F(16, TM_GLOBAL, false, false, false);                                      \
-      }                                                                                  \
-    }                                                                                    \
+    if (c.prec == 64) IMC_TRANSPORT_DISPATCH_P(F, 64);                                   \
+    else if (c.prec == 32) IMC_TRANSPORT_DISPATCH_P(F, 32);                              \
+    else IMC_TRANSPORT_DISPATCH_P(F, 16);                                                \
   } while (0)
 
 int transport_choose(const Ctx& c, int* grid_blocks, int* tally_mode, size_t* lds_bytes) {
   pick_tally(c, c.NB, tally_mode, lds_bytes);
-  if (use_surrogate(c) && *tally_mode == TM_WAVEROW) pick_tally(c, c.NB, tally_mode, lds_bytes);
-  if (use_surrogate(c) && *tally_mode == TM_WAVEROW) { *tally_mode = TM_GLOBAL; *lds_bytes = 0; }
+  if (use_surrogate(c) && (*tally_mode == TM_WAVEROW || *tally_mode == TM_HASH)) {
+    *tally_mode = TM_GLOBAL;
+    *lds_bytes = 0;
+  }
   *lds_bytes += cell_table_bytes(c);   // LDS layout: [tally][cell table][surrogate]
   const size_t total = *lds_bytes + (use_surrogate(c) ? SURR_LDS : 0);
   const int tm = *tally_mode;
@@ -1365,6 +1349,7 @@ __global__ __launch_bounds__(TPB) void k_clean_scatter(SoA src, SoA dst, int64_t
 
 int clean_choose(const Ctx& c, int* grid_blocks, int* tally_mode, size_t* lds_bytes) {
   pick_tally(c, c.C, tally_mode, lds_bytes);
+  if (*tally_mode == TM_HASH) { *tally_mode = TM_GLOBAL; *lds_bytes = 0; }   // census: global atomics
   *grid_blocks = 2 * c.num_cu;
   return 0;
 }

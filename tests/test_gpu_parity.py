@@ -156,20 +156,41 @@ def test_zero_variate_in_a_transparent_cell_gpu(prec):
     e.close()
 
 
-@pytest.mark.parametrize("mode", [1, 2, 3])
+@pytest.mark.parametrize("mode", [1, 2, 3, 4])
 @pytest.mark.parametrize("geom", [1, 2])
 def test_transport_replay_every_tally_mode(mode, geom):
-    """Per-wave LDS rows (1), one LDS row per block (2) and replicated global FP64 atomics (3)
-    give the oracle's tallies at FP64; blocks_per_cu=2 also exercises a small persistent grid."""
+    """Per-wave LDS rows (1), one LDS row per block (2), replicated global FP64 atomics (3) and
+    the hashed LDS table (4) give the oracle's tallies at FP64; blocks_per_cu=2 also exercises a
+    small persistent grid."""
     case = case1d(prec=64) if geom == 1 else case2d(prec=64)
     print(f"[tally mode] mode={mode} geometry={geom}", flush=True)
     e, op, st, rec, (dep, lost, segs, g), (o_dep, o_lost, o_segs, o_rec) = _replay(case, tally_mode=mode,
                                                                                      blocks_per_cu=2)
     tm = e.timing()["transport_tally_mode"]
-    assert tm == {1: 0, 2: 1, 3: 2}[mode]
+    assert tm == {1: 0, 2: 1, 3: 2, 4: 3}[mode]
     np.testing.assert_allclose(dep, o_dep, rtol=1e-9, atol=1e-9 * np.abs(o_dep).max())
     assert lost == pytest.approx(o_lost, rel=1e-9, abs=1e-15)
     assert np.count_nonzero(_alive(g) != _alive(o_rec)) == 0
+    e.close()
+
+
+@pytest.mark.parametrize("prec", [64, 32])
+def test_large_mesh_uses_the_hashed_tally(prec):
+    """100 x 90 cells: a block row does not fit 64 KiB of LDS at FP64 (9000 bins), so the automatic
+    choice is the hashed table (FP32 rows fit, so it is requested there); with more bins than its
+    4096 slots, colliding bins take the global path.  Tallies match the oracle (FP64 to 1e-9;
+    FP32 to the tolerances of the FP32 replay)."""
+    case = case2d(nx=100, ny=90, prec=prec)
+    print(f"[hashed tally] prec={prec}", flush=True)
+    tuning = {} if prec == 64 else {"tally_mode": 4}
+    e, op, st, rec, (dep, lost, segs, g), (o_dep, o_lost, o_segs, o_rec) = _replay(case, n_input=40000, **tuning)
+    assert e.timing()["transport_tally_mode"] == 3
+    if prec == 64:
+        np.testing.assert_allclose(dep, o_dep, rtol=1e-9, atol=1e-9 * np.abs(o_dep).max())
+        assert lost == pytest.approx(o_lost, rel=1e-9, abs=1e-15)
+        assert np.count_nonzero(_alive(g) != _alive(o_rec)) == 0
+    else:
+        assert np.abs(dep - o_dep).sum() <= 2e-3 * np.abs(o_dep).sum()
     e.close()
 
 
