@@ -52,7 +52,8 @@ typedef struct {
   int64_t capacity;          /* initial particle capacity; grows on demand */
   /* Tuning, 0 = automatic.  reserved[0]: transport tally mode (1 per-wave LDS rows,
      2 per-block LDS row, 3 global FP64 atomics, 4 hashed per-block LDS table).  reserved[1]: transport blocks per CU
-     (upper bound on the occupancy-derived persistent grid; 0 means 5).  Others must be 0. */
+     (upper bound on the occupancy-derived persistent grid; 0 means 5).  reserved[2]: log2
+     of the hashed tally's LDS slots, 8..13 (0 means 12).  Others must be 0. */
   int32_t reserved[8];
 } imc_config;
 

@@ -165,7 +165,7 @@ class Engine:
                  energyscales: Sequence[float] = (1.0,), bc=(VACUUM, VACUUM, VACUUM, VACUUM),
                  seed: int = 1234, phys_c: float = 1.0, phys_a: float = 1.0,
                  distancescale: float = 1.0, capacity: int = 1 << 16, quirks: int = 0,
-                 device: int = 0, tally_mode: int = 0, blocks_per_cu: int = 0):
+                 device: int = 0, tally_mode: int = 0, blocks_per_cu: int = 0, hash_bits: int = 0):
         L = lib()
         cfg = ImcConfig()
         cfg.geometry = geometry
@@ -187,6 +187,7 @@ class Engine:
         cfg.quirks = quirks
         cfg.reserved[0] = int(tally_mode)        # 0 auto, 1 wave rows, 2 block row, 3 global, 4 hashed
         cfg.reserved[1] = int(blocks_per_cu)     # 0 auto: occupancy-derived persistent grid
+        cfg.reserved[2] = int(hash_bits)         # 0 auto: log2 of the hashed tally's slots
         self.geometry = geometry
         self.precision = precision
         self.nx = nx

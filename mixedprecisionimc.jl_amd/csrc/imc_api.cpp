@@ -135,6 +135,7 @@ Geo make_geo(const Ctx& c) {
   g.ics = 1.0 / (c.cfg.phys_c * c.cfg.distancescale);
   g.cell_off = -1;
   g.gcopies = gtally_copies(c.NB);
+  g.hash_bits = hash_bits(c);
   for (int k = 0; k < 16; ++k) g.scales[k] = c.cfg.energyscales[k];
   return g;
 }

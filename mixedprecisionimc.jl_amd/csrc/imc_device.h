@@ -231,21 +231,22 @@ __device__ __forceinline__ void neumaier(double& s, double& c, double v) {
 //            bins do not fit in LDS).
 enum TallyMode { TM_WAVEROW = 0, TM_BLOCKROW = 1, TM_GLOBAL = 2, TM_HASH = 3 };
 
-// TM_HASH: an open LDS table of HASH_SLOTS (bin, value) slots per block for meshes whose bin
-// row does not fit in LDS.  A bin claims its slot (multiplicative hash) on first touch;
-// deposits to claimed bins are LDS atomics, a bin whose slot holds another bin goes to the
-// global copy, and the block adds every claimed slot to the global copy once at the end.
-constexpr int HASH_BITS = 12;
-constexpr int HASH_SLOTS = 1 << HASH_BITS;
-__device__ __forceinline__ uint32_t hash_slot(int bin) {
-  return ((uint32_t)bin * 2654435761u) >> (32 - HASH_BITS);
+// TM_HASH: an open LDS table of 2^hbits (bin, value) slots per block for meshes whose bin
+// row does not fit in LDS: values [2^hbits] then bin tags [2^hbits] (-1 = free).  A bin
+// claims its slot (multiplicative hash) on first touch; deposits to claimed bins are LDS
+// atomics, a bin whose slot holds another bin goes to the global copy, and the block adds
+// every claimed slot to the global copy once at the end.
+constexpr int HASH_BITS_MIN = 8, HASH_BITS_MAX = 13;
+__device__ __forceinline__ uint32_t hash_slot(int bin, int hbits) {
+  return ((uint32_t)bin * 2654435761u) >> (32 - hbits);
 }
 
 template <class Acc, int TM>
 struct Tally {
   Acc* lds;          // WAVEROW: [nwaves][NB][2]; BLOCKROW: [NB]
-  double* gbins;     // GLOBAL: [NB]
+  double* gbins;     // GLOBAL, HASH: [NB], this block's copy
   int NB;
+  int hbits;         // HASH: log2 of the table's slots
 
   // All 64 lanes must call this together.  bin < 0 marks "no deposit".
   __device__ __forceinline__ void add(int bin, Acc val) {
@@ -259,8 +260,8 @@ struct Tally {
     }
     if (TM == TM_HASH) {
       if (bin < 0) return;
-      int* tags = (int*)(lds + HASH_SLOTS);
-      const uint32_t s = hash_slot(bin);
+      int* tags = (int*)(lds + (1 << hbits));
+      const uint32_t s = hash_slot(bin, hbits);
       int t = tags[s];
       if (t == -1) {
         t = atomicCAS(&tags[s], -1, bin);

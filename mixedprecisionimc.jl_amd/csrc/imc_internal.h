@@ -72,6 +72,7 @@ struct Geo {                   // kernel view of the geometry and step coefficie
   double ics;                  // 1/(phys_c * distancescale)
   int cell_off;                // 1D: byte offset of the LDS cell table, -1 when cells are read from HBM
   int gcopies;                 // rows of the global-atomic tally
+  int hash_bits;               // log2 of the hashed tally's LDS slots
   double scales[16];
 };
 
@@ -168,6 +169,7 @@ hipError_t launch_clean(Ctx& c, int64_t n_total);
 hipError_t launch_reduce_partials(Ctx& c, int nblocks, int tally_mode, int nbins, double* out);
 hipError_t launch_zero(Ctx& c, void* p, size_t bytes);
 hipError_t launch_sum_copies(Ctx& c, const double* copies, int ncopies, int64_t n, double* out);
+int hash_bits(const Ctx& c);
 int transport_choose(const Ctx& c, int* grid_blocks, int* tally_mode, size_t* lds_bytes);
 int clean_choose(const Ctx& c, int* grid_blocks, int* tally_mode, size_t* lds_bytes);
 hipError_t surr_generate(Ctx& c, int64_t n, uint32_t seed, unsigned long long* d_events);

@@ -605,10 +605,11 @@ int64_t birth_hist_len(int64_t n_births) {
 // Tally flush helpers shared by transport and clean.
 // ---------------------------------------------------------------------------------
 template <class Acc, int TM>
-__device__ __forceinline__ void tally_init(Acc* lds, int NB) {
+__device__ __forceinline__ void tally_init(Acc* lds, int NB, int hbits = 0) {
   if (TM == TM_HASH) {
-    int* tags = (int*)(lds + HASH_SLOTS);
-    for (int i = threadIdx.x; i < HASH_SLOTS; i += blockDim.x) { lds[i] = (Acc)0; tags[i] = -1; }
+    const int slots = 1 << hbits;
+    int* tags = (int*)(lds + slots);
+    for (int i = threadIdx.x; i < slots; i += blockDim.x) { lds[i] = (Acc)0; tags[i] = -1; }
   } else if (TM == TM_WAVEROW) {
     for (int i = threadIdx.x; i < NWAVES * NB * 2; i += blockDim.x) lds[i] = (Acc)0;
   } else if (TM == TM_BLOCKROW) {
@@ -766,9 +767,9 @@ __global__ __launch_bounds__(TPB) void k_transport(SoA s, Geo g, Scalars* sc, vo
     }
     s_cell = w;
   }
-  tally_init<Acc, TM>(lds, NB);       // its barrier also publishes the cell table
+  tally_init<Acc, TM>(lds, NB, g.hash_bits);   // its barrier also publishes the cell table
   Tally<Acc, TM> tal{lds, (TM == TM_GLOBAL || TM == TM_HASH) ? gbins + (size_t)(blockIdx.x % g.gcopies) * NB : gbins,
-                     NB};
+                     NB, g.hash_bits};
 
   const T* __restrict__ dxs = (const T*)g.dxs;
   const T* __restrict__ dys = (const T*)g.dys;
@@ -1110,11 +1111,18 @@ __global__ __launch_bounds__(TPB) void k_transport(SoA s, Geo g, Scalars* sc, vo
   }
   if (TM == TM_HASH) {                    // claimed slots go to the block's global copy once
     __syncthreads();
-    const int* tags = (const int*)(lds + HASH_SLOTS);
-    for (int j = threadIdx.x; j < HASH_SLOTS; j += blockDim.x)
+    const int slots = 1 << g.hash_bits;
+    const int* tags = (const int*)(lds + slots);
+    for (int j = threadIdx.x; j < slots; j += blockDim.x)
       if (tags[j] >= 0) atomicAdd(&tal.gbins[tags[j]], (double)lds[j]);
   }
   tally_flush<Acc, TM>(lds, NB, (Acc*)partial_v);
+}
+
+// log2 of the hashed tally's slots: cfg.reserved[2] when set, else 12.
+int hash_bits(const Ctx& c) {
+  const int r = c.cfg.reserved[2];
+  return r >= HASH_BITS_MIN && r <= HASH_BITS_MAX ? r : 12;
 }
 
 // Tally mode: cfg.reserved[0] (0 auto, 1 WAVEROW, 2 BLOCKROW, 3 GLOBAL, 4 HASH).  Auto keeps
@@ -1126,7 +1134,7 @@ static void pick_tally(const Ctx& c, int nbins, int* tally_mode, size_t* lds_byt
   int req = c.cfg.reserved[0];
   if (req == 1 && wave_rows > 64 * 1024) req = 0;
   if (req == 2 && block_row > 64 * 1024) req = 0;
-  const size_t hash_bytes = (size_t)HASH_SLOTS * (acc + 4);
+  const size_t hash_bytes = ((size_t)1 << hash_bits(c)) * (acc + 4);
   if (req == 1) { *tally_mode = TM_WAVEROW; *lds_bytes = wave_rows; }
   else if (req == 3) { *tally_mode = TM_GLOBAL; *lds_bytes = 0; }
   else if (req == 4) { *tally_mode = TM_HASH; *lds_bytes = hash_bytes; }
