@@ -101,6 +101,10 @@ __device__ __forceinline__ void mlp_forward_wave(const uint16_t* __restrict__ wb
   const uint64_t lt = l == 0 ? 0ull : (~0ull >> (64 - l));
   const int rank = __popcll(mask & lt);
   for (int c = 0; 16 * c < nact; ++c) {                       // wave-uniform trip count
+    // The weight and bias addresses are made opaque per pass, so the compiler cannot hoist
+    // the 24 weight fragments and 36 biases out of the pass loop: held across the caller's
+    // event loop, they took ~84 VGPRs and halved the transport kernel's occupancy.
+    asm volatile("" : "+s"(W1t), "+s"(W2t), "+s"(W3t), "+s"(b1), "+s"(b2), "+s"(b3));
     const bool mine = active && (rank >> 4) == c;
     if (mine) {
       const int sidx = rank & 15;
