@@ -13,7 +13,7 @@ import time
 
 import numpy as np
 
-DEFAULT_BOX = {"tau_cell": (2.5, 200.0), "tau_b": (2.0, 1.0e4), "kappa": (1.0e-5, 4.0), "depth_min": 1.0}
+DEFAULT_BOX = {"tau_cell": (2.5, 200.0), "tau_b": (2.0, 1.0e4), "kappa": (1.0e-5, 4.0), "depth_min": 0.7}
 
 
 def train(particles, *, n_samples: int = 40_000_000, epochs: int = 60, batch: int = 8192, lr: float = 1.5e-3,
