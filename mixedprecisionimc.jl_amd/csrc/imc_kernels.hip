@@ -20,7 +20,11 @@ namespace imc {
 
 constexpr int TPB = 256;           // threads per block of the persistent kernels
 constexpr int NWAVES = TPB / 64;
-constexpr uint32_t CHUNK = 128;    // slots a wave claims from the work queue at a time
+// Slots a wave claims from the work queue at a time.  Plain transport: 128 (many short
+// histories; 64 and 32 were 6% and 30% slower on the Marshak wave from queue contention).  With
+// the surrogate: 32 (fewer, longer histories, where a wave's unstarted backlog lengthens the
+// tail when the queue runs dry; 7% faster than 128 on the thick/thin blocks).
+constexpr uint32_t CHUNK = 128, CHUNK_SURR = 32;
 constexpr int CLEAN_ITEMS = 8;     // slots per thread per clean tile
 constexpr int CLEAN_TILE = TPB * CLEAN_ITEMS;
 
@@ -681,8 +685,8 @@ hipError_t launch_reduce_partials(Ctx& c, int nblocks, int tally_mode, int nbins
 
 // ---------------------------------------------------------------------------------
 // Transport (imc_transport.jl:46-197 in 1D, :515-720 in 2D).
-// Persistent blocks; each wave pulls CHUNK-slot ranges from a device work counter and
-// refills a lane as soon as its history ends.  History in slot p uses Philox counters
+// Persistent blocks; each wave pulls CHUNK-slot (CHUNK_SURR with the surrogate) ranges from a
+// device work counter and refills a lane as soon as its history ends.  History in slot p uses Philox counters
 // (p, event, TAG_EVENT, redraw) keyed by (seed, step).
 //
 // SURR (1D): a lane whose cell has collision optical thickness sigma_c*dx inside the
@@ -846,11 +850,12 @@ __global__ __launch_bounds__(TPB) void k_transport(SoA s, Geo g, Scalars* sc, vo
     while (need != 0ull && !done) {
       if (cur >= cend) {
         uint32_t base = 0;
-        if (lane == 0) base = atomicAdd(&sc->work, CHUNK);
+        constexpr uint32_t chunk = SURR ? CHUNK_SURR : CHUNK;
+        if (lane == 0) base = atomicAdd(&sc->work, chunk);
         base = __shfl(base, 0, 64);
         if (base >= n) { done = true; break; }
         cur = base;
-        cend = min(base + CHUNK, n);
+        cend = min(base + chunk, n);
       }
       const uint32_t avail = cend - cur;
       const uint32_t rank = (uint32_t)__popcll(need & lt_mask);
