@@ -121,6 +121,55 @@ int imc_synchronize(imc_ctx* ctx);
 /* Bytes of particle state per particle held between kernels (S_state). */
 int imc_state_bytes(const imc_ctx* ctx, int32_t* bytes);
 
+/* ---- Device-resident step ----------------------------------------------------------
+ * The whole main-loop body (MixedPrecisionIMC.jl:137-149 in 1D, :166-174 in 2D) on the GPU in
+ * FP64 around the particle loop: Update.update (imc_update.jl:12-70), sourcing, transport,
+ * clean, Tally.tally (imc_tally.jl:11-75 and the census part :157-190) and the sums of
+ * EnergyCheck.energychecker (imc_energycheck.jl:10-38).  The material state (temperature,
+ * material energy density) stays on the device between steps; one wait per step returns it.
+ *
+ * imc_set_material gives the per-cell material description once (column-major, C cells):
+ * sigma = coef * T^power for absorption and scattering (mesh.sigma_a[:,2:3], imc_mesh.jl:136-142),
+ * bee (mesh.bee; unused when linearized), the initial temperature, radsource and the surface
+ * temperatures (laid out as in imc_set_step_fields).  It resets the step sequence. */
+typedef struct {
+  const double* sigma_a_coef;
+  const double* sigma_a_power;
+  const double* sigma_s_coef;
+  const double* sigma_s_power;
+  const double* bee;
+  const double* temp;
+  const double* radsource;
+  const double* t_surf[4];
+  double alpha;               /* constants.jl:11 */
+  int32_t linearized;         /* LINEARIZED = TRUE: bee = 4aT^3, beta = 1 (imc_update.jl:18-23) */
+  int32_t marshak;            /* 1D "MARSHAK WAVE": sigma_a = coef / T^3 (imc_update.jl:32-34) */
+} imc_material;
+
+/* Results of one device-resident step (the scalars of the energy check). */
+typedef struct {
+  double totalenergy;         /* mesh.totalenergy (emitted this step) */
+  double totalenergydep;      /* sum over cells and scales of energydep * volume / scale */
+  double radenergy;           /* sum over cells of radenergydens * volume */
+  double lostenergy;          /* this step's vacuum loss */
+  int64_t histories;          /* particles transported */
+  int64_t n_alive;            /* census population after clean */
+  int64_t segments;           /* event-loop passes (simvars.iterations increment) */
+} imc_step_result;
+
+int imc_set_material(imc_ctx* ctx, const imc_material* material);
+
+/* One step: step index (Philox key), dt, and the sourcing budget as in imc_source.  first != 0
+ * forms the initial material energy density first (imc_tally.jl:29-34, at t = 0).  On return temp,
+ * radenergydens and matenergydens (C each, may be NULL) hold the step's new values. */
+int imc_step(imc_ctx* ctx, int64_t step, double dt, int32_t first, int64_t n_input, int64_t n_max,
+             double cellmin, double* temp, double* radenergydens, double* matenergydens,
+             imc_step_result* result);
+
+/* The last imc_step's energy increase nrg_inc (C), which Tally.tally appends to
+ * mesh.energyincrease_saved (imc_tally.jl:46-58). */
+int imc_step_energy_increase(imc_ctx* ctx, double* energyincrease);
+
 /* ---- Learned thick-cell surrogate -------------------------------------------------
  * Fills the slot of Transport.MC_RW (imc_transport.jl:212-479, deck RANDOMWALK = TRUE):
  * a 6 -> 64 -> 64 -> 4 network (BF16 matrix-core inference and training, FP32

@@ -8,8 +8,11 @@ from __future__ import annotations
 import numpy as np
 
 
-def energychecker(inputs: dict, mesh, simvars, particles=None) -> float:
-    if mesh.geometry == "1D":
+def energychecker(inputs: dict, mesh, simvars, particles=None, radenergy=None) -> float:
+    """radenergy: the radiation energy when already summed (the device-resident step forms it)."""
+    if radenergy is not None:
+        radenergy = float(radenergy)
+    elif mesh.geometry == "1D":
         radenergy = float(np.sum(mesh.radenergydens * mesh.dx))
     else:
         radenergy = float(np.sum(mesh.radenergydens * mesh.dx[:, None] * mesh.dy[None, :]))

@@ -10,6 +10,10 @@ per step (MixedPrecisionIMC.jl:136-176).  Deck parsing, mesh, Fleck factor, temp
 update and energy check run on the host in FP64; sourcing, transport, clean and the census
 tally run in libimc_hip on the GPU.  ``particles`` is the device-resident population
 (an ``Engine``) instead of a Vector of records.
+
+``step_device`` (``run(..., device=True)``) runs the same loop body as one ``imc_step``: the
+FP64 update, temperature update and energy sums then run on the GPU as well, and the host
+keeps the clock, the saved histories and the energy check's final arithmetic.
 """
 from __future__ import annotations
 
@@ -22,7 +26,7 @@ import numpy as np
 from . import Clean, Constants, EnergyCheck, Input, Mesh, Output, Sourcing, Tally, Transport, Update, Utilities
 from ._hip import REFLECT, VACUUM, Engine, ImcError
 
-__all__ = ["main", "run", "timestep", "SimVars", "Engine", "ImcError", "Input", "Constants", "Mesh",
+__all__ = ["main", "run", "step", "step_device", "timestep", "SimVars", "Engine", "ImcError", "Input", "Constants", "Mesh",
            "Update", "Sourcing", "Transport", "Clean", "Tally", "EnergyCheck", "Output", "Utilities"]
 
 
@@ -136,10 +140,51 @@ def step(inputs: dict, mesh, simvars: SimVars, particles: Engine, timestepping: 
     return err
 
 
+def _set_device_material(inputs: dict, mesh, simvars: SimVars, particles: Engine) -> None:
+    """imc_set_material from the mesh: sigma = coef * T^power (mesh.sigma_*[..., 1:3]), bee,
+    temperature, radsource and surface temperatures, with the Update.update switches."""
+    if mesh.geometry == "1D":
+        tsurf = [np.array([mesh.temp_surf[0]]), np.array([mesh.temp_surf[1]])]
+    else:
+        tsurf = [np.asarray(v, dtype=np.float64) for v in mesh.temp_surf]
+    linearized = Input.get_scalar(inputs, "LINEARIZED") == "TRUE"
+    marshak = simvars.geometry == "1D" and Input.get_scalar(inputs, "NAME").upper() == "MARSHAK WAVE"
+    particles.set_material(mesh.sigma_a[..., 1], mesh.sigma_a[..., 2], mesh.sigma_s[..., 1], mesh.sigma_s[..., 2],
+                           None if linearized else mesh.bee, mesh.temp, mesh.radsource, tsurf, Constants.alpha,
+                           linearized, marshak)
+    particles.material_set = True
+
+
+def step_device(inputs: dict, mesh, simvars: SimVars, particles: Engine, timestepping: str) -> float:
+    """The loop body of ``step`` as one device-resident imc_step (Update.update, sourcing,
+    transport, clean, Tally.tally and the energy sums on the GPU); the host keeps the clock,
+    the saved histories and the energy check (imc_energycheck.jl:34).  The per-step coefficient
+    arrays (fleck, sigma_a[..., 0], energydep, emittedenergy) stay on the device."""
+    if not getattr(particles, "material_set", False):
+        _set_device_material(inputs, mesh, simvars, particles)
+    temp, rad, mat, r = particles.step(simvars.stepindex, simvars.dt, simvars.t == 0.0, simvars.n_input,
+                                       simvars.n_max, simvars.cellmin)
+    mesh.temp = temp.reshape(mesh.shape, order="F")
+    mesh.radenergydens = rad.reshape(mesh.shape, order="F")
+    mesh.matenergydens = mat.reshape(mesh.shape, order="F")
+    mesh.totalenergy = r["totalenergy"]
+    mesh.totalenergydep = r["totalenergydep"]
+    mesh.lostenergy += r["lostenergy"]
+    simvars.iterations += r["segments"]
+    mesh.energyincrease_saved.append(particles.step_energy_increase().reshape(mesh.shape, order="F"))
+    mesh.temp_saved.append(mesh.temp.copy())
+    mesh.matenergy_saved.append(mesh.matenergydens.copy())
+    mesh.radenergy_saved.append(mesh.radenergydens.copy())
+    err = EnergyCheck.energychecker(inputs, mesh, simvars, particles, radenergy=r["radenergy"])
+    Output.plotting(inputs, mesh, simvars)
+    timestep(timestepping, simvars)
+    return err
+
+
 def run(deck, *, precision: Optional[int] = None, seed: Optional[int] = None, max_steps: int = 0,
-        quirks: int = 0, verbose: bool = False, overrides: Optional[dict] = None):
+        quirks: int = 0, verbose: bool = False, overrides: Optional[dict] = None, device: bool = False):
     """Runs a deck (path or parsed dict) to ENDTIME (or ``max_steps``) and returns
-    (mesh, simvars, particles, info)."""
+    (mesh, simvars, particles, info).  device=True runs each step as one imc_step."""
     inputs = Input.readInputs(deck) if isinstance(deck, str) else dict(deck)
     if overrides:
         inputs.update(overrides)
@@ -148,7 +193,7 @@ def run(deck, *, precision: Optional[int] = None, seed: Optional[int] = None, ma
     t0 = time.perf_counter()
     nsteps = 0
     while simvars.t <= simvars.t_end and (max_steps <= 0 or nsteps < max_steps):
-        err = step(inputs, mesh, simvars, particles, timestepping)
+        err = (step_device if device else step)(inputs, mesh, simvars, particles, timestepping)
         nsteps += 1
         if verbose:
             print(f"step {nsteps} t={simvars.t:.6g} n={len(particles)} energy error {err:.3e}", flush=True)

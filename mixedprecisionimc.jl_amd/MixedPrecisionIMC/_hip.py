@@ -78,6 +78,20 @@ class SurrogateBox(ctypes.Structure):
                 ("depth_min", ctypes.c_float)]
 
 
+class ImcMaterial(ctypes.Structure):
+    _fields_ = [("sigma_a_coef", ctypes.POINTER(ctypes.c_double)), ("sigma_a_power", ctypes.POINTER(ctypes.c_double)),
+                ("sigma_s_coef", ctypes.POINTER(ctypes.c_double)), ("sigma_s_power", ctypes.POINTER(ctypes.c_double)),
+                ("bee", ctypes.POINTER(ctypes.c_double)), ("temp", ctypes.POINTER(ctypes.c_double)),
+                ("radsource", ctypes.POINTER(ctypes.c_double)), ("t_surf", ctypes.POINTER(ctypes.c_double) * 4),
+                ("alpha", ctypes.c_double), ("linearized", ctypes.c_int32), ("marshak", ctypes.c_int32)]
+
+
+class ImcStepResult(ctypes.Structure):
+    _fields_ = [("totalenergy", ctypes.c_double), ("totalenergydep", ctypes.c_double),
+                ("radenergy", ctypes.c_double), ("lostenergy", ctypes.c_double),
+                ("histories", ctypes.c_int64), ("n_alive", ctypes.c_int64), ("segments", ctypes.c_int64)]
+
+
 SURROGATE_NPARAM = 4868
 
 
@@ -105,6 +119,10 @@ SYMBOLS = {
     "imc_reset_timing": (ctypes.c_int, [_VP]),
     "imc_synchronize": (ctypes.c_int, [_VP]),
     "imc_state_bytes": (ctypes.c_int, [_VP, ctypes.POINTER(ctypes.c_int32)]),
+    "imc_set_material": (ctypes.c_int, [_VP, ctypes.POINTER(ImcMaterial)]),
+    "imc_step": (ctypes.c_int, [_VP, ctypes.c_int64, ctypes.c_double, ctypes.c_int32, ctypes.c_int64, ctypes.c_int64,
+                                ctypes.c_double, _DP, _DP, _DP, ctypes.POINTER(ImcStepResult)]),
+    "imc_step_energy_increase": (ctypes.c_int, [_VP, _DP]),
     "imc_surrogate_configure": (ctypes.c_int, [_VP, ctypes.POINTER(SurrogateBox)]),
     "imc_surrogate_init": (ctypes.c_int, [_VP, ctypes.c_uint32]),
     "imc_surrogate_generate": (ctypes.c_int, [_VP, ctypes.c_int64, ctypes.c_uint32, _I64P]),
@@ -295,6 +313,40 @@ class Engine:
         b = ctypes.c_int32()
         self._check(lib().imc_state_bytes(self._h, ctypes.byref(b)))
         return b.value
+
+    # -- device-resident step ---------------------------------------------------
+    def set_material(self, sigma_a_coef, sigma_a_power, sigma_s_coef, sigma_s_power, bee, temp, radsource,
+                     t_surf, alpha: float, linearized: bool, marshak: bool):
+        """imc_set_material: per-cell material description for imc_step (column-major)."""
+        keep = [_f64(a) for a in (sigma_a_coef, sigma_a_power, sigma_s_coef, sigma_s_power, temp, radsource)]
+        bee_a = _f64(bee) if bee is not None else None
+        ts = [_f64(np.atleast_1d(a)) for a in t_surf]
+        while len(ts) < 4:
+            ts.append(np.zeros(1))
+        m = ImcMaterial()
+        m.sigma_a_coef, m.sigma_a_power, m.sigma_s_coef, m.sigma_s_power = (_dp(a) for a in keep[:4])
+        m.temp, m.radsource = _dp(keep[4]), _dp(keep[5])
+        m.bee = _dp(bee_a) if bee_a is not None else _DP()
+        for q in range(4):
+            m.t_surf[q] = _dp(ts[q])
+        m.alpha = float(alpha)
+        m.linearized = int(bool(linearized))
+        m.marshak = int(bool(marshak))
+        self._check(lib().imc_set_material(self._h, ctypes.byref(m)))
+
+    def step(self, step: int, dt: float, first: bool, n_input: int, n_max: int, cellmin: float):
+        """imc_step: one device-resident step.  Returns (temp, radenergydens, matenergydens, result)."""
+        temp, rad, mat = np.zeros(self.C), np.zeros(self.C), np.zeros(self.C)
+        r = ImcStepResult()
+        self._check(lib().imc_step(self._h, int(step), float(dt), int(bool(first)), int(n_input), int(n_max),
+                                   float(cellmin), _dp(temp), _dp(rad), _dp(mat), ctypes.byref(r)))
+        self.n = r.n_alive
+        return temp, rad, mat, {name: getattr(r, name) for name, _ in ImcStepResult._fields_}
+
+    def step_energy_increase(self) -> np.ndarray:
+        inc = np.zeros(self.C)
+        self._check(lib().imc_step_energy_increase(self._h, _dp(inc)))
+        return inc
 
     # -- learned thick-cell surrogate (slot of Transport.MC_RW) -----------------
     def surrogate_configure(self, tau_cell=(20.0, 200.0), tau_b=(2.0, 1e4), kappa=(1e-5, 4.0), depth_min=3.0):

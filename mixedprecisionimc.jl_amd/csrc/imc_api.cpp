@@ -174,6 +174,7 @@ int imc_create(imc_ctx** out, const imc_config* cfg) {
   HIPCHK(hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking), "stream");
   HIPCHK(hipEventCreate(&c.ev0), "event"); HIPCHK(hipEventCreate(&c.ev1), "event");
   HIPCHK(hipEventCreate(&c.evk0), "event"); HIPCHK(hipEventCreate(&c.evk1), "event");
+  for (hipEvent_t& e : c.evs) HIPCHK(hipEventCreate(&e), "event");
   const size_t T = tsize(c.prec), W = wsize(c.prec), C = (size_t)c.C;
   const size_t nx = (size_t)c.nx, ny = (size_t)c.ny;
   bool ok = true;
@@ -238,10 +239,12 @@ int imc_destroy(imc_ctx* h) {
   void* ptrs[] = {c.d_dx, c.d_dy, c.d_dxs, c.d_dys, c.d_dxw, c.d_dyw, c.d_fields, c.d_sigcol, c.d_sigdep, c.d_isig, c.d_idxw, c.d_idyw, c.d_ent_e, c.d_ent_esc, c.d_ent_term,
                   c.d_ent_k, c.d_ent_cnt, c.d_ent_off, c.d_red, c.d_emitted, c.d_dep, c.d_gtally, c.d_rad, c.d_sc,
                   c.d_partial, c.d_tile, c.d_bhist, c.d_surr_feat, c.d_surr_targ, c.d_surr_params, c.d_surr_grad,
-                  c.d_surr_m, c.d_surr_v, c.d_surr_wbf, c.d_surr_bias, c.d_surr_loss, c.d_surr_events, c.d_surr_cells};
+                  c.d_surr_m, c.d_surr_v, c.d_surr_wbf, c.d_surr_bias, c.d_surr_loss, c.d_surr_events, c.d_surr_cells,
+                  c.d_mat};
   for (void* p : ptrs) (void)hipFree(p);
   if (c.h_fields) (void)hipHostFree(c.h_fields);
-  hipEvent_t evs[] = {c.ev0, c.ev1, c.evk0, c.evk1, c.ev_fields};
+  if (c.h_out) (void)hipHostFree(c.h_out);
+  hipEvent_t evs[] = {c.ev0, c.ev1, c.evk0, c.evk1, c.ev_fields, c.evs[0], c.evs[1], c.evs[2], c.evs[3]};
   for (hipEvent_t e : evs)
     if (e) (void)hipEventDestroy(e);
   if (c.stream) (void)hipStreamDestroy(c.stream);
@@ -385,8 +388,8 @@ int imc_transport(imc_ctx* h, double* energydep, double* lostenergy, int64_t* se
   int grid, tm; size_t lds;
   transport_choose(c, &grid, &tm, &lds);
   HIPCHK(hipEventRecord(c.ev0, c.stream), "event");
-  HIPCHK(launch_zero(c, (char*)c.d_sc + offsetof(Scalars, segments), sizeof(Scalars) - offsetof(Scalars, segments)),
-         "zero scalars");
+  HIPCHK(launch_zero(c, (char*)c.d_sc + offsetof(Scalars, segments),
+                     offsetof(Scalars, prep_underflow) - offsetof(Scalars, segments)), "zero scalars");
   if (tm == TM_GLOBAL || tm == TM_HASH) HIPCHK(launch_zero(c, c.d_gtally, 8 * (size_t)c.NB * gtally_copies(c.NB)), "zero tallies");
   HIPCHK(hipEventRecord(c.evk0, c.stream), "event");
   HIPCHK(launch_transport(c, grid, tm, lds), "transport kernel");
@@ -579,6 +582,195 @@ int imc_synchronize(imc_ctx* h) {
 int imc_state_bytes(const imc_ctx* h, int32_t* bytes) {
   if (!h || !bytes) return IMC_EINVAL;
   *bytes = state_bytes(h->c.prec, h->c.is2d);
+  return IMC_OK;
+}
+
+// ---------------------------------------------------------------------------------
+// device-resident step
+// ---------------------------------------------------------------------------------
+int imc_set_material(imc_ctx* h, const imc_material* m) {
+  if (!h) return IMC_EINVAL;
+  Ctx& c = h->c;
+  if (!m) return fail(c, IMC_EINVAL, "material required");
+  if (!c.geo_set) return fail(c, IMC_EINVAL, "imc_set_geometry must be called first");
+  if (!m->sigma_a_coef || !m->sigma_a_power || !m->sigma_s_coef || !m->sigma_s_power || !m->temp ||
+      !m->radsource || (!m->linearized && !m->bee))
+    return fail(c, IMC_EINVAL, "null material array");
+  if (!std::isfinite(m->alpha)) return fail(c, IMC_EINVAL, "alpha must be finite");
+  const size_t C = (size_t)c.C;
+  for (size_t i = 0; i < C; ++i) {
+    const double v[6] = {m->sigma_a_coef[i], m->sigma_a_power[i], m->sigma_s_coef[i], m->sigma_s_power[i],
+                         m->temp[i], m->radsource[i]};
+    for (double x : v)
+      if (!std::isfinite(x)) return fail(c, IMC_ENUMERIC, "non-finite material coefficient or temperature");
+    if (!m->linearized && !(std::isfinite(m->bee[i]) && m->bee[i] != 0.0))
+      return fail(c, IMC_ENUMERIC, "bee must be finite and non-zero");
+  }
+  std::vector<double> ts;
+  if (c.is2d) {
+    for (int q = 0; q < 4; ++q) {
+      if (!m->t_surf[q]) return fail(c, IMC_EINVAL, "null t_surf entry");
+      ts.insert(ts.end(), m->t_surf[q], m->t_surf[q] + (q < 2 ? c.nx : c.ny));
+    }
+  } else {
+    if (!m->t_surf[0] || !m->t_surf[1]) return fail(c, IMC_EINVAL, "null t_surf entry");
+    ts = {m->t_surf[0][0], m->t_surf[1][0]};
+  }
+  if (!c.d_mat) {
+    if (dalloc(&c.d_mat, 8 * MAT_N * C) != hipSuccess) return fail(c, IMC_ENOMEM, "material allocation failed");
+    if (hipHostMalloc((void**)&c.h_out, 8 * 4 * C, hipHostMallocDefault) != hipSuccess) {
+      c.h_out = nullptr;
+      return fail(c, IMC_ENOMEM, "pinned result allocation failed");
+    }
+  }
+  std::vector<double> st(MAT_N * C, 0.0);
+  const double* src[6] = {m->sigma_a_coef, m->sigma_a_power, m->sigma_s_coef, m->sigma_s_power,
+                          m->linearized ? nullptr : m->bee, m->temp};
+  const int slot[6] = {MAT_SA_COEF, MAT_SA_POW, MAT_SS_COEF, MAT_SS_POW, MAT_BEE, MAT_TEMP};
+  for (int q = 0; q < 6; ++q)
+    if (src[q]) std::memcpy(st.data() + slot[q] * C, src[q], 8 * C);
+  HIPCHK(hipStreamSynchronize(c.stream), "material sync");
+  HIPCHK(hipMemcpy(c.d_mat, st.data(), 8 * MAT_N * C, hipMemcpyHostToDevice), "material H2D");
+  HIPCHK(hipMemcpy(c.d_radsrc, m->radsource, 8 * C, hipMemcpyHostToDevice), "radsource H2D");
+  HIPCHK(hipMemcpy(c.d_tsurf, ts.data(), 8 * ts.size(), hipMemcpyHostToDevice), "t_surf H2D");
+  c.mat_linearized = m->linearized != 0;
+  c.mat_marshak = m->marshak != 0;
+  c.mat_alpha = m->alpha;
+  c.mat_set = true;
+  return IMC_OK;
+}
+
+int imc_step(imc_ctx* h, int64_t step, double dt, int32_t first, int64_t n_input, int64_t n_max, double cellmin,
+             double* temp, double* radenergydens, double* matenergydens, imc_step_result* res) {
+  if (!h) return IMC_EINVAL;
+  Ctx& c = h->c;
+  if (!c.mat_set) return fail(c, IMC_EINVAL, "imc_set_material must be called before imc_step");
+  if (step < 0 || step > 0xFFFFFFFFll) return fail(c, IMC_EINVAL, "step out of range");
+  if (!(dt > 0.0) || !std::isfinite(dt)) return fail(c, IMC_EINVAL, "dt must be positive and finite");
+  if (n_input < 0 || n_max < 0 || !(cellmin >= 0.0)) return fail(c, IMC_EINVAL, "negative particle budget");
+  c.step = step;
+  c.dt = dt;
+  const size_t C = (size_t)c.C;
+  // Update.update on the device, then the per-cell tables of the tracking type
+  HIPCHK(hipEventRecord(c.evs[0], c.stream), "event");
+  HIPCHK(launch_zero(c, (char*)c.d_sc + offsetof(Scalars, prep_underflow),
+                     offsetof(Scalars, pad_) - offsetof(Scalars, prep_underflow)), "zero flags");
+  HIPCHK(launch_update(c, first != 0), "update kernel");
+  HIPCHK(launch_prep(c), "prep kernel");
+  if (c.prec == 16) {                   // binary16 can round 1/sigma_c to 0: refuse before sourcing
+    int rc = fetch_scalars(c);
+    if (rc) return rc;
+    if (c.h_sc.field_error) return fail(c, IMC_ENUMERIC, "non-finite or out-of-range coefficients after the update");
+    if (c.h_sc.prep_underflow)
+      return fail(c, IMC_ENUMERIC, (std::to_string(c.h_sc.prep_underflow) + " cell(s) have a collision mean free "
+                  "path 1/sigma_c that rounds to 0 in the tracking precision; use a wider PRECISION").c_str());
+  }
+  c.step_set = true;
+  // sourcing, as imc_source, without its closing wait when the birth bound fits
+  HIPCHK(hipEventRecord(c.ev0, c.stream), "event");
+  HIPCHK(launch_source_entries(c), "source entries kernel");
+  HIPCHK(launch_tree_sum(c, c.d_ent_term, c.NE, (double*)((char*)c.d_sc + offsetof(Scalars, E_tot))),
+         "energy sum kernel");
+  const int64_t n_census = c.h_sc.n_census;
+  const double budget = std::ceil(std::max((double)n_input, cellmin));
+  const double bound_d = budget * ((c.cfg.quirks & IMC_QUIRK_Q1_RADSRC_NBODY) ? 2.0 : 1.0) +
+                         (double)c.NE * (std::ceil(cellmin) + 1.0) + 16.0;
+  const int64_t bound = bound_d < 4e18 ? (int64_t)bound_d : INT64_MAX / 4;
+  const bool direct = n_census + bound <= c.buf[c.cur].cap && birth_hist_len(bound) <= c.bhist_len;
+  HIPCHK(launch_counts_and_scan(c, n_input, n_max, cellmin, direct ? c.buf[c.cur].cap : INT64_MAX), "count kernels");
+  int64_t n_bound;                      // the population transport and clean will see, or a bound of it
+  if (direct) {
+    HIPCHK(launch_births(c, bound), "birth kernel");
+    n_bound = n_census + bound;
+  } else {
+    int rc = fetch_scalars(c);
+    if (rc) return rc;
+    if (c.h_sc.field_error) return fail(c, IMC_ENUMERIC, "non-finite or out-of-range coefficients after the update");
+    if (!std::isfinite(c.h_sc.E_tot) || !(c.h_sc.E_tot > 0.0))
+      return fail(c, IMC_ENUMERIC, "total source energy is zero or not finite");
+    rc = grow(c, c.h_sc.n_census + c.h_sc.n_births, c.h_sc.n_census);
+    if (rc) return rc;
+    const int64_t hl = birth_hist_len(c.h_sc.n_births);
+    if (hl > c.bhist_len) {
+      (void)hipFree(c.d_bhist);
+      c.d_bhist = nullptr;
+      c.bhist_len = 0;
+      if (dalloc(&c.d_bhist, sizeof(int64_t) * (size_t)(hl + hl / 4)) != hipSuccess)
+        return fail(c, IMC_ENOMEM, "birth-order scratch allocation failed");
+      c.bhist_len = hl + hl / 4;
+    }
+    HIPCHK(launch_births(c, c.h_sc.n_births), "birth kernel");
+    n_bound = c.h_sc.n_census + c.h_sc.n_births;
+  }
+  HIPCHK(hipEventRecord(c.ev1, c.stream), "event");
+  // transport
+  int grid, tm; size_t lds;
+  transport_choose(c, &grid, &tm, &lds);
+  HIPCHK(launch_zero(c, (char*)c.d_sc + offsetof(Scalars, segments),
+                     offsetof(Scalars, prep_underflow) - offsetof(Scalars, segments)), "zero scalars");
+  if (tm == TM_GLOBAL || tm == TM_HASH)
+    HIPCHK(launch_zero(c, c.d_gtally, 8 * (size_t)c.NB * gtally_copies(c.NB)), "zero tallies");
+  HIPCHK(hipEventRecord(c.evk0, c.stream), "event");
+  HIPCHK(launch_transport(c, grid, tm, lds), "transport kernel");
+  HIPCHK(hipEventRecord(c.evk1, c.stream), "event");
+  if (tm == TM_GLOBAL || tm == TM_HASH)
+    HIPCHK(launch_sum_copies(c, c.d_gtally, gtally_copies(c.NB), c.NB, c.d_dep), "tally copies");
+  else HIPCHK(launch_reduce_partials(c, grid, tm, c.NB, c.d_dep), "tally reduction");
+  HIPCHK(hipEventRecord(c.evs[1], c.stream), "event");
+  // clean with the census tally, then the material update and the energy-check sums
+  HIPCHK(launch_clean(c, n_bound), "clean kernels");
+  HIPCHK(hipEventRecord(c.evs[2], c.stream), "event");
+  HIPCHK(launch_temperature(c), "temperature kernel");
+  HIPCHK(launch_step_sums(c), "energy sums kernel");
+  const int out_slot[4] = {MAT_TEMP, -1, MAT_E, MAT_INC};
+  for (int q = 0; q < 4; ++q) {
+    const double* from = out_slot[q] < 0 ? c.d_rad : c.d_mat + out_slot[q] * C;
+    HIPCHK(hipMemcpyAsync(c.h_out + q * C, from, 8 * C, hipMemcpyDeviceToHost, c.stream), "results D2H");
+  }
+  HIPCHK(hipMemcpyAsync(&c.h_sc, c.d_sc, sizeof(Scalars), hipMemcpyDeviceToHost, c.stream), "scalars D2H");
+  HIPCHK(hipEventRecord(c.evs[3], c.stream), "event");
+  HIPCHK(hipStreamSynchronize(c.stream), "step sync");
+  c.rad_valid = false;
+  c.timing.ms_source += elapsed(c.evs[0], c.ev1);
+  c.timing.ms_transport += elapsed(c.ev1, c.evs[1]);
+  c.timing.ms_transport_kernel += elapsed(c.evk0, c.evk1);
+  c.timing.ms_clean += elapsed(c.evs[1], c.evs[2]);
+  c.timing.ms_tally += elapsed(c.evs[2], c.evs[3]);
+  c.timing.n_transport_launches += 1;
+  c.timing.histories += c.h_sc.n_cleaned;
+  c.timing.census_out += (int64_t)c.h_sc.census_out;
+  c.timing.segments += (int64_t)c.h_sc.segments;
+  c.timing.surrogate_events += (int64_t)c.h_sc.surr_events;
+  c.timing.truncated += (int64_t)c.h_sc.truncated;
+  c.timing.transport_tally_mode = tm;
+  c.timing.transport_grid = grid;
+  if (c.h_sc.field_error) return fail(c, IMC_ENUMERIC, "non-finite or out-of-range coefficients after the update");
+  if (c.h_sc.prep_underflow)
+    return fail(c, IMC_ENUMERIC, "a collision mean free path rounds to 0 in the tracking precision");
+  if (!std::isfinite(c.h_sc.E_tot) || !(c.h_sc.E_tot > 0.0))
+    return fail(c, IMC_ENUMERIC, "total source energy is zero or not finite");
+  if (c.h_sc.birth_overflow) return fail(c, IMC_ENOMEM, "births exceed the particle buffer");
+  double* outs[4] = {temp, radenergydens, matenergydens, nullptr};
+  for (int q = 0; q < 3; ++q)
+    if (outs[q]) std::memcpy(outs[q], c.h_out + q * C, 8 * C);
+  if (res) {
+    res->totalenergy = c.h_sc.E_tot;
+    res->totalenergydep = c.h_sc.E_dep;
+    res->radenergy = c.h_sc.E_rad;
+    res->lostenergy = c.h_sc.E_lost;
+    res->histories = c.h_sc.n_cleaned;
+    res->n_alive = c.h_sc.n_alive;
+    res->segments = (int64_t)c.h_sc.segments;
+  }
+  return IMC_OK;
+}
+
+int imc_step_energy_increase(imc_ctx* h, double* energyincrease) {
+  if (!h) return IMC_EINVAL;
+  Ctx& c = h->c;
+  if (!c.mat_set || !c.h_out) return fail(c, IMC_EINVAL, "no device-resident step has run");
+  if (!energyincrease) return fail(c, IMC_EINVAL, "energyincrease required");
+  std::memcpy(energyincrease, c.h_out + 3 * (size_t)c.C, 8 * (size_t)c.C);
   return IMC_OK;
 }
 

@@ -46,6 +46,13 @@ struct Scalars {
   uint32_t truncated;          // histories ended by the per-history event cap
   uint32_t prep_underflow;     // cells whose 1/sigma_c rounds to 0 in the tracking type
   uint32_t birth_overflow;     // the step's births did not fit the particle buffer
+  // device-resident step (imc_step): written by its kernels, not cleared by transport
+  uint32_t field_error;        // cells whose updated coefficients are non-finite or out of range
+  uint32_t pad_;
+  double E_dep;                // mesh.totalenergydep
+  double E_rad;                // radiation energy (sum of radenergydens * volume)
+  double E_lost;               // the step's vacuum loss
+  int64_t n_cleaned;           // population handed to the last clean (the step's histories)
 };
 
 struct SurrBox {               // training box of the thick-cell surrogate (see imc_mlp.h)
@@ -149,10 +156,21 @@ struct Ctx {
   std::vector<double> h_dep, h_rad;
   bool rad_valid = false;
 
+  // device-resident step (imc_set_material / imc_step): FP64 material state per cell
+  bool mat_set = false;
+  int mat_linearized = 0, mat_marshak = 0;
+  double mat_alpha = 0.0;
+  double* d_mat = nullptr;         // [sa_coef, sa_pow, ss_coef, ss_pow, bee, temp, matE, nrg_inc] x C
+  double* h_out = nullptr;         // pinned: temp, radenergydens, matenergydens, nrg_inc (C each)
+
   // timing
   hipEvent_t ev0 = nullptr, ev1 = nullptr, evk0 = nullptr, evk1 = nullptr;
+  hipEvent_t evs[4] = {nullptr, nullptr, nullptr, nullptr};   // imc_step stage boundaries
   imc_timing timing{};
 };
+
+// Material state offsets inside Ctx::d_mat (multiples of C).
+enum MatSlot { MAT_SA_COEF = 0, MAT_SA_POW, MAT_SS_COEF, MAT_SS_POW, MAT_BEE, MAT_TEMP, MAT_E, MAT_INC, MAT_N };
 
 Geo make_geo(const Ctx& c);
 int state_bytes(int prec, bool is2d);
@@ -165,13 +183,17 @@ hipError_t launch_counts_and_scan(Ctx& c, int64_t n_input, int64_t n_max, double
 hipError_t launch_births(Ctx& c, int64_t n_births);
 int64_t birth_hist_len(int64_t n_births);
 hipError_t launch_transport(Ctx& c, int grid_blocks, int tally_mode, size_t lds_bytes);
-hipError_t launch_clean(Ctx& c, int64_t n_total);
+hipError_t launch_clean(Ctx& c, int64_t n_total);   // n_total: the count or an upper bound of it
 hipError_t launch_reduce_partials(Ctx& c, int nblocks, int tally_mode, int nbins, double* out);
 hipError_t launch_zero(Ctx& c, void* p, size_t bytes);
 hipError_t launch_sum_copies(Ctx& c, const double* copies, int ncopies, int64_t n, double* out);
 int hash_bits(const Ctx& c);
 int transport_choose(const Ctx& c, int* grid_blocks, int* tally_mode, size_t* lds_bytes);
 int clean_choose(const Ctx& c, int* grid_blocks, int* tally_mode, size_t* lds_bytes);
+// Device-resident step (imc_step.hip).
+hipError_t launch_update(Ctx& c, int first);
+hipError_t launch_temperature(Ctx& c);
+hipError_t launch_step_sums(Ctx& c);
 hipError_t surr_generate(Ctx& c, int64_t n, uint32_t seed, unsigned long long* d_events);
 hipError_t surr_make_weights(Ctx& c);
 hipError_t surr_grad(Ctx& c, const float* feat, const float* targ, int64_t n0, int64_t n, float inv_batch,

@@ -1273,14 +1273,18 @@ hipError_t launch_transport(Ctx& c, int grid, int tm, size_t lds) {
 
 // ---------------------------------------------------------------------------------
 // Clean (imc_clean.jl:6-19) fused with the census tally (imc_tally.jl:157-190).
+// The launch covers n slots, the population or an upper bound of it; the kernels stop at the
+// device's population (n_total for the count, n_cleaned -- the same value, kept by the scan,
+// which replaces n_total with the survivors -- for the scatter).
 //   k_clean_count  : alive count per tile of CLEAN_TILE slots
 //   k_clean_scan   : exclusive scan of tile counts (single block); n_alive
 //   k_clean_scatter: stable scatter of the live particles to the other buffer while
 //                    tallying E/(dx*scale) (1D) or E/(dx*dy*scale) (2D) per cell
 // ---------------------------------------------------------------------------------
-__global__ __launch_bounds__(TPB) void k_clean_count(const uint32_t* __restrict__ ck, int64_t n,
-                                                     int64_t* __restrict__ tile_cnt) {
+__global__ __launch_bounds__(TPB) void k_clean_count(const uint32_t* __restrict__ ck, int64_t n_bound,
+                                                     const Scalars* __restrict__ sc, int64_t* __restrict__ tile_cnt) {
   __shared__ int wsum[NWAVES];
+  const int64_t n = min(n_bound, sc->n_total);
   const int64_t base = (int64_t)blockIdx.x * CLEAN_TILE;
   int cnt = 0;
 #pragma unroll
@@ -1325,6 +1329,7 @@ __global__ __launch_bounds__(1024) void k_clean_scan(int64_t* __restrict__ tiles
     __syncthreads();
   }
   if (tid == 0) {
+    sc->n_cleaned = sc->n_total;
     sc->n_alive = running;
     sc->n_census = running;
     sc->n_births = 0;
@@ -1333,9 +1338,10 @@ __global__ __launch_bounds__(1024) void k_clean_scan(int64_t* __restrict__ tiles
 }
 
 template <int P, int TM, bool TWO_D>
-__global__ __launch_bounds__(TPB) void k_clean_scatter(SoA src, SoA dst, int64_t n, int64_t ntiles,
+__global__ __launch_bounds__(TPB) void k_clean_scatter(SoA src, SoA dst, int64_t n_bound, int64_t ntiles,
                                                        const int64_t* __restrict__ tile_off, Geo g,
-                                                       void* partial_v, double* gbins) {
+                                                       void* partial_v, double* gbins, const Scalars* __restrict__ sc) {
+  const int64_t n = min(n_bound, sc->n_cleaned);   // k_clean_scan has already set n_total to the survivors
   using T = typename Prec<P>::T;
   using W = typename Prec<P>::W;
   using Acc = typename Prec<P>::Acc;
@@ -1398,7 +1404,7 @@ int clean_choose(const Ctx& c, int* grid_blocks, int* tally_mode, size_t* lds_by
 hipError_t launch_clean(Ctx& c, int64_t n_total) {
   const int64_t ntiles = (n_total + CLEAN_TILE - 1) / CLEAN_TILE;
   if (ntiles > 0) {
-    k_clean_count<<<(unsigned)ntiles, TPB, 0, c.stream>>>(c.buf[c.cur].cellk, n_total, c.d_tile);
+    k_clean_count<<<(unsigned)ntiles, TPB, 0, c.stream>>>(c.buf[c.cur].cellk, n_total, c.d_sc, c.d_tile);
     hipError_t err = hipGetLastError();
     if (err != hipSuccess) return err;
   }
@@ -1415,7 +1421,7 @@ hipError_t launch_clean(Ctx& c, int64_t n_total) {
   SoA src = c.buf[c.cur], dst = c.buf[c.cur ^ 1];
 #define CL(P, TM_, D)                                                                                \
   k_clean_scatter<P, TM_, D><<<grid, TPB, lds, c.stream>>>(src, dst, n_total, ntiles, c.d_tile, g, \
-                                                          c.d_partial, c.d_rad)
+                                                          c.d_partial, c.d_rad, c.d_sc)
 #define CLD(P, TM_) do { if (c.is2d) CL(P, TM_, true); else CL(P, TM_, false); } while (0)
 #define CLT(P) do { if (tm == TM_WAVEROW) CLD(P, TM_WAVEROW);                                     \
                     else if (tm == TM_BLOCKROW) CLD(P, TM_BLOCKROW);                              \

@@ -131,6 +131,38 @@ def test_abi_library_exports_every_declared_symbol():
     assert set(declared) <= set(_hip.SYMBOLS)
 
 
+def test_abi_struct_layouts_match_the_c_header(tmp_path):
+    """sizeof and field offsets of every struct in include/imc_hip.h, as gcc lays them out,
+    against the ctypes mirrors."""
+    import shutil
+    import subprocess
+    if shutil.which("gcc") is None:
+        pytest.skip("gcc not available")
+    structs = {"imc_config": _hip.ImcConfig, "imc_timing": _hip.ImcTiming,
+               "imc_surrogate_box": _hip.SurrogateBox, "imc_material": _hip.ImcMaterial,
+               "imc_step_result": _hip.ImcStepResult}
+    lines = ["#include <stdio.h>", "#include <stddef.h>", '#include "imc_hip.h"', "int main(void) {"]
+    for cname, py in structs.items():
+        lines.append(f'  printf("{cname} sizeof %zu\\n", sizeof({cname}));')
+        for fname, _ in py._fields_:
+            lines.append(f'  printf("{cname} {fname} %zu\\n", offsetof({cname}, {fname}));')
+    lines += ["  return 0;", "}"]
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines) + "\n")
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-std=c99", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)], check=True)
+    out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split("\n")
+    got = {}
+    for line in out:
+        if line:
+            cname, fname, val = line.split()
+            got[(cname, fname)] = int(val)
+    for cname, py in structs.items():
+        assert got[(cname, "sizeof")] == ctypes.sizeof(py), cname
+        for fname, _ in py._fields_:
+            assert got[(cname, fname)] == getattr(py, fname).offset, (cname, fname)
+
+
 def test_abi_config_struct_layout():
     # imc_config as declared in include/imc_hip.h: 10 int32 + 2 int32 + 16 doubles + 3 doubles + int64 + 8 int32
     assert ctypes.sizeof(_hip.ImcConfig) == 4 * 12 + 8 * 19 + 8 + 4 * 8
