@@ -4,9 +4,11 @@ Workload (the metric's configuration): nonlinear Marshak wave in 1D, x in [0,0.2
 uniform cells, sigma_a = 10 T^-3, bee = 0.3, left surface T = 1, ramped dt (1e-4 x 1.1 up to
 1e-3), NINPUT 2e6, NMAX 1e7; FP32 tracking with compensated FP32 tallies (decks/c2_marshak_fp32.txt).
 
-A step is one IMC timestep: host FP64 update, GPU sourcing, transport, clean and census
-tally, host temperature update and energy check.  W warmup steps (the census builds to
-~NMAX), then K timed steps.  value = histories transported in the K steps / wall time of
+A step is one IMC timestep, run as one device-resident imc_step: FP64 update, sourcing,
+transport, clean, census tally, FP64 temperature update and the energy-check sums on the GPU,
+then the host's energy check and clock.  W warmup steps (the census builds to ~NMAX), then K
+timed steps (`host_update_step` repeats the window with the update, temperature update and
+energy check in numpy on the host).  value = histories transported in the K steps / wall time of
 the K steps.  The roofline line is for the transport kernel (the dominant kernel), timed
 with HIP events on the engine's stream: algorithmic bytes per launch =
 S_state x (histories + census survivors).  The CPU baseline is the oracle (a C++ port of the
@@ -77,17 +79,39 @@ def gpu_run(inputs, warmup: int, steps: int, precision=None):
     mesh, simvars, particles = M.setup(inputs, precision=precision)
     ts = str(inputs["TIMESTEPPING"]).upper()
     for _ in range(warmup):
-        M.step(inputs, mesh, simvars, particles, ts)
+        M.step_device(inputs, mesh, simvars, particles, ts)
     particles.synchronize()
     particles.reset_timing()
     done = 0
     while done < steps and simvars.t <= simvars.t_end:
-        M.step(inputs, mesh, simvars, particles, ts)
+        M.step_device(inputs, mesh, simvars, particles, ts)
         done += 1
     particles.synchronize()
     tm = particles.timing()
     particles.close()
     return tm["histories"] / _hot_s(tm), tm, done
+
+
+def host_update_step(inputs, warmup: int, steps: int):
+    """The same window with Update.update, Tally.tally and the energy check in numpy FP64 on the
+    host and one field upload per step (the reference's module split), beside `value`, which
+    runs each step as one device-resident imc_step."""
+    import MixedPrecisionIMC as M
+
+    mesh, simvars, particles = M.setup(inputs)
+    ts = str(inputs["TIMESTEPPING"]).upper()
+    for _ in range(warmup):
+        M.step(inputs, mesh, simvars, particles, ts)
+    particles.synchronize()
+    particles.reset_timing()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        M.step(inputs, mesh, simvars, particles, ts)
+    particles.synchronize()
+    wall = time.perf_counter() - t0
+    tm = particles.timing()
+    particles.close()
+    return {"histories_per_s": tm["histories"] / wall, "ms_per_step": 1e3 * wall / max(steps, 1)}
 
 
 def cpu_baseline(inputs, mesh, warmup: int, steps: int, gpu_same):
@@ -149,7 +173,7 @@ def _run_c4(prec, n_input, seed, steps, params=None, train=False):
         Surrogate.load(particles, params)
     particles.reset_timing()
     for _ in range(steps):
-        M.step(inp, mesh, simvars, particles, "CONSTANT")
+        M.step_device(inp, mesh, simvars, particles, "CONSTANT")
     tm = particles.timing()
     particles.close()
     hot_s = (tm["ms_source"] + tm["ms_transport"] + tm["ms_clean"] + tm["ms_tally"]) / 1e3
@@ -272,7 +296,7 @@ def main():
     mesh, simvars, particles = M.setup(inputs, tuning={"blocks_per_cu": args.blocks_per_cu})
     timestepping = Input.get_scalar(inputs, "TIMESTEPPING").upper()
     for _ in range(args.warmup):
-        M.step(inputs, mesh, simvars, particles, timestepping)
+        M.step_device(inputs, mesh, simvars, particles, timestepping)
     particles.synchronize()
     particles.reset_timing()
     t0 = time.perf_counter()
@@ -281,7 +305,7 @@ def main():
     for _ in range(args.steps):
         if simvars.t > simvars.t_end:
             break
-        M.step(inputs, mesh, simvars, particles, timestepping)
+        M.step_device(inputs, mesh, simvars, particles, timestepping)
         done += 1
         if done == args.cpu_steps and not args.no_cpu:
             t_snap = particles.timing()      # GPU figures on the CPU baseline's sample
@@ -328,6 +352,7 @@ def main():
     print(f"[bench] {done} steps, {hist} histories, {tm['segments']} segments, wall {wall:.3f}s, "
           f"transport kernel {tm['ms_transport_kernel']:.1f} ms over {launches} launches", file=sys.stderr, flush=True)
     particles.close()
+    line["host_update_step"] = host_update_step(inputs, args.warmup, done)
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(inputs, mesh, args.warmup, args.cpu_steps, snap)
         print(f"[bench] cpu baseline {line['cpu_baseline']['value']:.4g} histories/s", file=sys.stderr, flush=True)
