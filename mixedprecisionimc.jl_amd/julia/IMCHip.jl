@@ -16,6 +16,16 @@
 #         timestep(timestepping, simvars)
 #     end
 #
+# or, with the update and the temperature update on the GPU as well (one wait per step):
+#
+#     IMCHip.set_material!(eng, inputs, mesh, simvars)
+#     while simvars.t <= simvars.t_end
+#         IMCHip.step!(eng, mesh, simvars)              # replaces update, sourcing, MC, clean, tally
+#         EnergyCheck.energychecker(inputs, mesh, simvars, eng)
+#         Output.plotting(inputs, mesh, simvars)
+#         timestep(timestepping, simvars)
+#     end
+#
 # Arrays cross the boundary as Float64 (column-major, as the reference stores them).
 # This file is not exercised in this repository's test suite (no Julia toolchain is
 # available where it is built); the same C ABI is exercised through ctypes by tests/.
@@ -178,6 +188,77 @@ function tally!(inputs, mesh, simvars, e::Engine)
     push!(mesh.radenergy_saved, copy(mesh.radenergydens))
     e.step += 1
     return nothing
+end
+
+# The device-resident step: one imc_step replaces update!/sourcing!/transport!/clean!/tally!
+# (src/MixedPrecisionIMC.jl:137-146); the caller keeps energychecker's final arithmetic
+# (its radenergy is returned here) and timestep.
+struct ImcMaterial
+    sigma_a_coef::Ptr{Float64}
+    sigma_a_power::Ptr{Float64}
+    sigma_s_coef::Ptr{Float64}
+    sigma_s_power::Ptr{Float64}
+    bee::Ptr{Float64}
+    temp::Ptr{Float64}
+    radsource::Ptr{Float64}
+    t_surf::NTuple{4,Ptr{Float64}}
+    alpha::Float64
+    linearized::Int32
+    marshak::Int32
+end
+
+struct ImcStepResult
+    totalenergy::Float64
+    totalenergydep::Float64
+    radenergy::Float64
+    lostenergy::Float64
+    histories::Int64
+    n_alive::Int64
+    segments::Int64
+end
+
+function set_material!(e::Engine, inputs, mesh, simvars)
+    col(a, k) = Float64.(vec(e.geometry == 1 ? a[:, k] : a[:, :, k]))
+    sa, sap, ss, ssp = col(mesh.sigma_a, 2), col(mesh.sigma_a, 3), col(mesh.sigma_s, 2), col(mesh.sigma_s, 3)
+    bee = Float64.(vec(mesh.bee .* ones(size(mesh.temp))))
+    te = Float64.(vec(mesh.temp))
+    rs = Float64.(vec(mesh.radsource))
+    ts = e.geometry == 1 ? [Float64[mesh.temp_surf[1]], Float64[mesh.temp_surf[2]], [0.0], [0.0]] :
+         [Float64.(vec(mesh.temp_surf[i])) for i in 1:4]
+    lin = inputs["LINEARIZED"] == "TRUE"
+    marshak = e.geometry == 1 && uppercase(inputs["NAME"]) == "MARSHAK WAVE"
+    GC.@preserve sa sap ss ssp bee te rs ts begin
+        m = ImcMaterial(pointer(sa), pointer(sap), pointer(ss), pointer(ssp), pointer(bee), pointer(te),
+                        pointer(rs), ntuple(i -> pointer(ts[i]), 4),
+                        Float64(Main.MixedPrecisionIMC.Constants.alpha), Int32(lin), Int32(marshak))
+        check(e, ccall((:imc_set_material, LIB), Cint, (Ptr{Cvoid}, Ref{ImcMaterial}), e.ptr, m))
+    end
+    return nothing
+end
+
+function step!(e::Engine, mesh, simvars)
+    C = e.nx * e.ny
+    temp, rad, mat, inc = zeros(C), zeros(C), zeros(C), zeros(C)
+    r = Ref{ImcStepResult}()
+    check(e, ccall((:imc_step, LIB), Cint,
+                   (Ptr{Cvoid}, Int64, Float64, Int32, Int64, Int64, Float64, Ptr{Float64}, Ptr{Float64},
+                    Ptr{Float64}, Ref{ImcStepResult}),
+                   e.ptr, e.step, Float64(simvars.dt), Int32(simvars.t == 0.0), simvars.n_input, simvars.n_max,
+                   Float64(simvars.cellmin), temp, rad, mat, r))
+    check(e, ccall((:imc_step_energy_increase, LIB), Cint, (Ptr{Cvoid}, Ptr{Float64}), e.ptr, inc))
+    shape(v) = e.geometry == 1 ? v : reshape(v, e.nx, e.ny)
+    mesh.temp, mesh.radenergydens, mesh.matenergydens = shape(temp), shape(rad), shape(mat)
+    mesh.totalenergy = r[].totalenergy
+    mesh.totalenergydep = r[].totalenergydep
+    mesh.lostenergy += r[].lostenergy
+    simvars.iterations += r[].segments
+    push!(mesh.energyincrease_saved, shape(inc))
+    push!(mesh.temp_saved, copy(mesh.temp))
+    push!(mesh.matenergy_saved, copy(mesh.matenergydens))
+    push!(mesh.radenergy_saved, copy(mesh.radenergydens))
+    e.n = r[].n_alive
+    e.step += 1
+    return r[].radenergy
 end
 
 end # module
