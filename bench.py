@@ -280,6 +280,7 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=12)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-surrogate", action="store_true")
+    ap.add_argument("--no-host-update", action="store_true", help="skip the host-update comparison loop")
     ap.add_argument("--blocks-per-cu", type=int, default=0, help="transport grid bound (0: automatic)")
     ap.add_argument("--precision-study", action="store_true",
                     help="also run the FP16/FP32/FP64 tally-error study on the 2D channel deck")
@@ -352,7 +353,8 @@ def main():
     print(f"[bench] {done} steps, {hist} histories, {tm['segments']} segments, wall {wall:.3f}s, "
           f"transport kernel {tm['ms_transport_kernel']:.1f} ms over {launches} launches", file=sys.stderr, flush=True)
     particles.close()
-    line["host_update_step"] = host_update_step(inputs, args.warmup, done)
+    if not args.no_host_update:
+        line["host_update_step"] = host_update_step(inputs, args.warmup, done)
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(inputs, mesh, args.warmup, args.cpu_steps, snap)
         print(f"[bench] cpu baseline {line['cpu_baseline']['value']:.4g} histories/s", file=sys.stderr, flush=True)
