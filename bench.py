@@ -159,14 +159,16 @@ def _c4(prec, n_input):
     return inp
 
 
-def _run_c4(prec, n_input, seed, steps, params=None, train=False):
+def _run_c4(prec, n_input, seed, steps, params=None, train=False, random_walk=False):
     import MixedPrecisionIMC as M
     from MixedPrecisionIMC import Surrogate
 
     inp = _c4(prec, n_input)
     mesh, simvars, particles = M.setup(inp, seed=seed)
     info = None
-    if train:
+    if random_walk:
+        particles.random_walk_enable(True)
+    elif train:
         info = Surrogate.train(particles, seed=seed)
         params = particles.surrogate_params()
     elif params is not None:
@@ -181,8 +183,9 @@ def _run_c4(prec, n_input, seed, steps, params=None, train=False):
 
 
 def surrogate_leg(steps=20, n_input=400000, seeds=4):
-    """Thick/thin blocks workload: FP32 plain vs FP32 + learned surrogate at equal tally
-    error against an FP64 GPU oracle (plain FP64, 4x the particles, 4 seeds).
+    """Thick/thin blocks workload: FP32 plain vs FP32 + learned surrogate (and vs the analytic
+    random walk, the non-learned filling of the same slot, r_min 5) at equal tally error
+    against an FP64 GPU oracle (plain FP64, 4x the particles, 4 seeds).
     eps = relative RMS over cells of the final material temperature; FOM = 1/(eps^2 t)."""
     import numpy as np
 
@@ -190,12 +193,14 @@ def surrogate_leg(steps=20, n_input=400000, seeds=4):
     out = {}
     params = None
     train_info = None
-    for label in ("plain", "surrogate"):
+    for label in ("plain", "surrogate", "random_walk"):
         eps2, t, hist, segs, nsurr = [], 0.0, 0, 0, 0
         for s in range(seeds):
-            temp, hot, tm, params_out, info = _run_c4(32, n_input, 100 * (label == "surrogate") + s, steps,
+            seed = {"plain": 0, "surrogate": 100, "random_walk": 200}[label] + s
+            temp, hot, tm, params_out, info = _run_c4(32, n_input, seed, steps,
                                                       params=params if label == "surrogate" else None,
-                                                      train=(label == "surrogate" and params is None))
+                                                      train=(label == "surrogate" and params is None),
+                                                      random_walk=(label == "random_walk"))
             if label == "surrogate" and params is None:
                 params, train_info = params_out, info
             eps2.append(float(np.mean(((temp - ref) / ref) ** 2)))
@@ -209,6 +214,7 @@ def surrogate_leg(steps=20, n_input=400000, seeds=4):
                       "fom": 1.0 / (e2 * t / seeds)}
     out["fom_speedup"] = out["surrogate"]["fom"] / out["plain"]["fom"]
     out["time_speedup"] = out["plain"]["gpu_s_per_run"] / out["surrogate"]["gpu_s_per_run"]
+    out["random_walk_fom_speedup"] = out["random_walk"]["fom"] / out["plain"]["fom"]
     out["workload"] = (f"thick/thin blocks 1D (decks/c4_thickthin_fp32.txt), {steps} steps, NINPUT {n_input}, "
                        f"{seeds} seeds each; oracle: FP64 plain, 4x particles, {seeds} seeds")
     if train_info:

@@ -198,6 +198,17 @@ int imc_surrogate_gradient(imc_ctx* ctx, const float* features, const float* tar
                            float* grad, float* loss);
 int imc_surrogate_enable(imc_ctx* ctx, int32_t on);
 
+/* ---- Analytic random walk ----------------------------------------------------------
+ * The other filling of the Transport.MC_RW slot (imc_transport.jl:212-479, with its
+ * tables P_r / randomwalk_table at :734-797), corrected: in a cell, a particle whose distance
+ * R0 to the nearest face is at least r_min collision mean free paths (the reference: one
+ * total mean free path) replaces its walk inside [x - R0, x + R0] by the first exit of
+ * one-dimensional diffusion with D = c / (3 sigma_c), sigma_c = sigma_a (1 - f) + sigma_s
+ * (the reference used a sphere law and dropped sigma_s).  The weight follows the exact path
+ * length and every deposit is E_in - E_out, so energy is conserved (the reference's is not,
+ * quirk Q6).  Enabling it disables the surrogate and the other way round.  1D only. */
+int imc_random_walk_enable(imc_ctx* ctx, int32_t on, double r_min);
+
 #ifdef __cplusplus
 }
 #endif

@@ -26,7 +26,10 @@ def MC2D(mesh, simvars, particles: Engine) -> None:
 
 
 def MC_RW(mesh, simvars, rwvars, particles: Engine) -> None:
-    """The RANDOMWALK slot (imc_transport.jl:212-479).  The reference's analytic random walk
-    does not conserve energy; this build runs the plain event loop here, which is the
-    method the random walk approximates."""
+    """The RANDOMWALK slot (imc_transport.jl:212-479).  The reference's random walk does not
+    conserve energy (quirk Q6); here the slot is filled by the engine's corrected analytic
+    random walk (imc_random_walk_enable) unless the learned surrogate has been enabled
+    (Surrogate.train / Surrogate.load), which then takes its place."""
+    if not getattr(particles, "surrogate_on", False) and not getattr(particles, "random_walk_on", False):
+        particles.random_walk_enable(True)
     _run(mesh, simvars, particles)

@@ -162,6 +162,9 @@ def step_device(inputs: dict, mesh, simvars: SimVars, particles: Engine, timeste
     arrays (fleck, sigma_a[..., 0], energydep, emittedenergy) stay on the device."""
     if not getattr(particles, "material_set", False):
         _set_device_material(inputs, mesh, simvars, particles)
+    if (str(inputs.get("RANDOMWALK", "FALSE")).strip().upper() == "TRUE" and simvars.geometry == "1D"
+            and not getattr(particles, "surrogate_on", False) and not getattr(particles, "random_walk_on", False)):
+        particles.random_walk_enable(True)          # the MC_RW slot, as Transport.MC_RW fills it
     temp, rad, mat, r = particles.step(simvars.stepindex, simvars.dt, simvars.t == 0.0, simvars.n_input,
                                        simvars.n_max, simvars.cellmin)
     mesh.temp = temp.reshape(mesh.shape, order="F")

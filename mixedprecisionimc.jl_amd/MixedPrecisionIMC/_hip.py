@@ -138,6 +138,7 @@ SYMBOLS = {
                                               ctypes.c_int64, ctypes.POINTER(ctypes.c_float),
                                               ctypes.POINTER(ctypes.c_float)]),
     "imc_surrogate_enable": (ctypes.c_int, [_VP, ctypes.c_int32]),
+    "imc_random_walk_enable": (ctypes.c_int, [_VP, ctypes.c_int32, ctypes.c_double]),
 }
 
 _FP = ctypes.POINTER(ctypes.c_float)
@@ -398,3 +399,14 @@ class Engine:
 
     def surrogate_enable(self, on: bool = True):
         self._check(lib().imc_surrogate_enable(self._h, int(bool(on))))
+        self.surrogate_on = bool(on)
+        if on:
+            self.random_walk_on = False
+
+    def random_walk_enable(self, on: bool = True, r_min: float = 5.0):
+        """imc_random_walk_enable: the analytic random walk in cells at least r_min collision
+        mean free paths deep (the Transport.MC_RW slot); disables the surrogate."""
+        self._check(lib().imc_random_walk_enable(self._h, int(bool(on)), float(r_min)))
+        self.random_walk_on = bool(on)
+        if on:
+            self.surrogate_on = False

@@ -240,7 +240,7 @@ int imc_destroy(imc_ctx* h) {
                   c.d_ent_k, c.d_ent_cnt, c.d_ent_off, c.d_red, c.d_emitted, c.d_dep, c.d_gtally, c.d_rad, c.d_sc,
                   c.d_partial, c.d_tile, c.d_bhist, c.d_surr_feat, c.d_surr_targ, c.d_surr_params, c.d_surr_grad,
                   c.d_surr_m, c.d_surr_v, c.d_surr_wbf, c.d_surr_bias, c.d_surr_loss, c.d_surr_events, c.d_surr_cells,
-                  c.d_mat};
+                  c.d_mat, c.d_rw_table};
   for (void* p : ptrs) (void)hipFree(p);
   if (c.h_fields) (void)hipHostFree(c.h_fields);
   if (c.h_out) (void)hipHostFree(c.h_out);
@@ -965,6 +965,8 @@ int imc_surrogate_gradient(imc_ctx* h, const float* features, const float* targe
   return IMC_OK;
 }
 
+static int fit_partials(Ctx& c);
+
 int imc_surrogate_enable(imc_ctx* h, int32_t on) {
   if (!h) return IMC_EINVAL;
   Ctx& c = h->c;
@@ -972,7 +974,67 @@ int imc_surrogate_enable(imc_ctx* h, int32_t on) {
   int rc = surr_alloc(c);
   if (rc) return rc;
   c.surr_enabled = on != 0;
-  // partial buffer must cover the transport grid of the active configuration
+  if (on) c.rw_enabled = false;
+  return fit_partials(c);
+}
+
+// First-exit law of one-dimensional diffusion from (-1, 1) started at 0, in the dimensionless
+// time a = D t / R0^2: F(a) = P(exit by a).  Small a: F = 2 sum_n (-1)^n erfc((2n+1) / (2 sqrt a))
+// (images of the two absorbing ends); large a: F = 1 - (4/pi) sum_k (-1)^k/(2k+1)
+// exp(-(2k+1)^2 pi^2 a / 4) (eigenfunction expansion).  Both converge fast on their side of 0.25.
+static double rw_exit_cdf(double a) {
+  if (a <= 0.0) return 0.0;
+  double f = 0.0;
+  if (a < 0.25) {
+    const double r = 1.0 / (2.0 * std::sqrt(a));
+    for (int n = 0; n < 8; ++n) f += (n & 1 ? -2.0 : 2.0) * std::erfc((2 * n + 1) * r);
+  } else {
+    const double pi = 3.14159265358979323846;
+    double s = 0.0;
+    for (int k = 0; k < 40; ++k) {
+      const double m = 2 * k + 1;
+      s += (k & 1 ? -1.0 : 1.0) / m * std::exp(-m * m * pi * pi * a / 4.0);
+    }
+    f = 1.0 - 4.0 / pi * s;
+  }
+  return std::min(std::max(f, 0.0), 1.0);
+}
+
+// Transport.MC_RW (imc_transport.jl:212-479) with the random-walk table of :786-797, as a
+// correct analytic walk (see the RW note in imc_kernels.hip).
+int imc_random_walk_enable(imc_ctx* h, int32_t on, double r_min) {
+  if (!h) return IMC_EINVAL;
+  Ctx& c = h->c;
+  if (on && c.is2d) return fail(c, IMC_EINVAL, "the random walk is available for 1D geometry");
+  if (on && !(r_min >= 1.0 && std::isfinite(r_min)))
+    return fail(c, IMC_EINVAL, "r_min must be at least one mean free path");
+  if (on && !c.d_rw_table) {
+    // inverse CDF at u = i / N by bisection; the tail past (N-1)/N uses the closed form
+    std::vector<float> tab(RW_TABLE_POINTS);
+    for (int i = 0; i < RW_TABLE_POINTS; ++i) {
+      const double u = (double)i / RW_TABLE_POINTS;
+      double lo = 0.0, hi = 60.0;
+      for (int it = 0; it < 100; ++it) {
+        const double mid = 0.5 * (lo + hi);
+        if (rw_exit_cdf(mid) < u) lo = mid; else hi = mid;
+      }
+      tab[i] = (float)(0.5 * (lo + hi));
+    }
+    if (dalloc(&c.d_rw_table, sizeof(float) * RW_TABLE_POINTS) != hipSuccess)
+      return fail(c, IMC_ENOMEM, "random-walk table allocation failed");
+    HIPCHK(hipMemcpy(c.d_rw_table, tab.data(), sizeof(float) * RW_TABLE_POINTS, hipMemcpyHostToDevice),
+           "random-walk table H2D");
+  }
+  c.rw_enabled = on != 0;
+  if (on) {
+    c.rw_min = r_min;
+    c.surr_enabled = false;
+  }
+  return fit_partials(c);
+}
+
+// The tally partial buffer must cover the transport grid of the active configuration.
+static int fit_partials(Ctx& c) {
   int grid, tm; size_t lds;
   transport_choose(c, &grid, &tm, &lds);
   const size_t acc = c.prec == 64 ? 8 : 4;

@@ -55,6 +55,8 @@ struct Scalars {
   int64_t n_cleaned;           // population handed to the last clean (the step's histories)
 };
 
+constexpr int RW_TABLE_POINTS = 4096;   // analytic random walk: inverse-CDF table of the exit time
+
 struct SurrBox {               // training box of the thick-cell surrogate (see imc_mlp.h)
   float tau_cell_min, tau_cell_max;
   float tau_b_min, tau_b_max;
@@ -151,6 +153,11 @@ struct Ctx {
   unsigned long long* d_surr_events = nullptr;
   void* d_surr_cells = nullptr;      // [C] float4 per-cell decision inputs
   int surr_step = 0;
+
+  // analytic random walk (imc_random_walk_enable); table of RW_TABLE_POINTS values
+  bool rw_enabled = false;
+  double rw_min = 5.0;
+  float* d_rw_table = nullptr;       // [4096] inverse CDF of the dimensionless exit time
 
   // host copies of the last transport tallies and census tally
   std::vector<double> h_dep, h_rad;

@@ -707,6 +707,8 @@ struct SurrArgs {
   SurrBox box;
   SurrMap map;
   int tally_bytes;
+  const float* rw_table; // random walk: inverse CDF of the exit time (RW_TABLE_POINTS values)
+  float rw_min;          // random walk: smallest half-width, in collision mean free paths
 };
 
 // Per-cell inputs of the surrogate decision, formed once per transport launch.
@@ -772,13 +774,56 @@ __device__ __forceinline__ float surr_exit_mu(float u) {
   return fminf(m, 1.0f);
 }
 
-template <int P, int TM, bool TWO_D, bool SURR, bool CELL>
+// ---------------------------------------------------------------------------------
+// RW (1D): the analytic random walk of the Transport.MC_RW slot (imc_transport.jl:212-479,
+// corrected).  A lane whose distance R0 to the nearest face is at least rw_min collision mean
+// free paths replaces its walk inside [x - R0, x + R0] by the diffusion limit of effective
+// scattering: D = c / (3 sigma_c) with the collision cross section sigma_c = sigma_a (1 - f) +
+// sigma_s (the reference omitted sigma_s).  In path units the dimensionless time is
+// a = s / (3 sigma_c R0^2); the first exit from the interval has the law of one-dimensional
+// diffusion started at its centre (an exact table of its inverse CDF, with the closed-form tail
+// a = -(4/pi^2) ln(pi (1 - u) / 4)), and the walker leaves at x - R0 or x + R0 with equal
+// probability -- one of them is the cell face -- in an outward direction from mu (1 + 1.5 mu).
+// A walker still inside at census is placed by the interval's conditional density: the
+// fundamental mode cos(pi y / 2) for a >= 0.2, else a Gaussian of variance 2a with the
+// first-order images, accepted with probability 1 - e^{(y-1)/a} - e^{-(y+1)/a}.  The weight
+// follows the exact path length (E exp(-f sigma_a s)), the deposit is E_in - E_out, and the 1%
+// cut-off applies at its own path length, so every history conserves energy.
+// ---------------------------------------------------------------------------------
+constexpr uint32_t TAG_RW = 5u;
+
+__device__ __forceinline__ float rw_exit_a(const float* __restrict__ tab, float u) {
+  const float f = u * (float)RW_TABLE_POINTS;
+  const int i = (int)f;
+  if (i >= RW_TABLE_POINTS - 1) return -0.40528473f * __logf(0.78539816f * fmaxf(1.0f - u, 0x1p-30f));
+  const float w = f - (float)i;
+  return tab[i] + w * (tab[i + 1] - tab[i]);
+}
+
+__device__ __forceinline__ float rw_census_y(float a, float u, uint32_t slot, uint32_t ev, uint32_t seed,
+                                             uint32_t step) {
+  if (a >= 0.2f) return 0.63661977f * asinf(2.0f * u - 1.0f);
+  const float sg = sqrtf(2.0f * a);
+  float y = 0.0f;
+  for (uint32_t at = 1; at <= 16; ++at) {
+    const u4 r = philox(slot, ev, TAG_RW, at, seed, step);
+    const float z = sqrtf(-2.0f * __logf((float)((r.r[0] >> 8) + 1) * 0x1p-24f)) * __cosf(6.2831853f * u24(r.r[1]));
+    const float yc = sg * z;
+    if (fabsf(yc) >= 1.0f) continue;
+    y = yc;
+    if (u24(r.r[2]) < 1.0f - __expf((yc - 1.0f) / a) - __expf(-(yc + 1.0f) / a)) break;
+  }
+  return y;
+}
+
+template <int P, int TM, bool TWO_D, int ACC, bool CELL>
 __global__ __launch_bounds__(TPB) void k_transport(SoA s, Geo g, Scalars* sc, void* partial_v,
                                                    double* gbins, SurrArgs sa) {
   using T = typename Prec<P>::T;
   using W = typename Prec<P>::W;
   using Acc = typename Prec<P>::Acc;
-  static_assert(!(SURR && TWO_D), "the surrogate path is 1D");
+  constexpr bool SURR = ACC == 1, RW = ACC == 2;       // learned surrogate / analytic random walk
+  static_assert(!((SURR || RW) && TWO_D), "the surrogate and the random walk are 1D");
   static_assert(!(CELL && TWO_D), "the LDS cell table is 1D");
   extern __shared__ __align__(16) unsigned char smem[];
   Acc* lds = (Acc*)smem;
@@ -851,7 +896,7 @@ __global__ __launch_bounds__(TPB) void k_transport(SoA s, Geo g, Scalars* sc, vo
     while (need != 0ull && !done) {
       if (cur >= cend) {
         uint32_t base = 0;
-        constexpr uint32_t chunk = SURR ? CHUNK_SURR : CHUNK;
+        constexpr uint32_t chunk = (SURR || RW) ? CHUNK_SURR : CHUNK;
         if (lane == 0) base = atomicAdd(&sc->work, chunk);
         base = __shfl(base, 0, 64);
         if (base >= n) { done = true; break; }
@@ -916,7 +961,91 @@ __global__ __launch_bounds__(TPB) void k_transport(SoA s, Geo g, Scalars* sc, vo
       }
     }
 
-    if (SURR && want) {
+    // ---- analytic random walk (1D, at least rw_min mean free paths from both faces) ----
+    bool rw = false;
+    T rwR0 = zero;
+    if (RW && has) {
+      const T cdx_ = dx_of(ix);
+      const T cis_ = CELL ? s_cell[ix].isig : isig[ix];
+      rwR0 = (x < cdx_ - x) ? x : cdx_ - x;
+      rw = (float)rwR0 > 0.0f && (float)rwR0 >= sa.rw_min * (float)cis_;
+    }
+
+    if (RW && rw) {
+      ++segs;
+      ++nsurr;
+      const u4 q = philox(slot, ev, TAG_RW, 0u, g.seed, g.step);
+      bin = k * C + ix;
+      const T cdx_ = dx_of(ix);
+      float isf, sdf;
+      W cidx_;
+      if (CELL) {
+        const CellRec<P> r = s_cell[ix];
+        isf = (float)r.isig; sdf = (float)r.sigdep; cidx_ = r.idxw;
+      } else {
+        isf = (float)isig[ix]; sdf = (float)sigdep[ix]; cidx_ = idxw[ix];
+      }
+      const float R0 = (float)rwR0;
+      const float spa = 3.0f * R0 * R0 / isf;                  // path length per unit of a
+      const float s_e = rw_exit_a(sa.rw_table, u24(q.r[0])) * spa;
+      const float s_rem = (float)(cT * (dtT - t) * sT);
+      const float s_cut = sdf > 0.f ? __logf((float)E / (float)Emin) / sdf : INFINITY;
+      if (s_cut < fminf(s_e, s_rem)) {                        // 1% cut-off first: all deposited
+        val = (Acc)(E * cidx_);
+        CK[slot] = DEAD;
+        has = false;
+      } else if (s_e < s_rem) {                                // leaves the interval
+        const W Enew = E * (W)__expf(-sdf * s_e);
+        val = (Acc)((E - Enew) * cidx_);
+        E = Enew;
+        t = t + Tm<T>::from((double)s_e) * icsT;
+        const bool right = u24(q.r[1]) < 0.5f;
+        const float m = surr_exit_mu(u24(q.r[2]));
+        mu = Tm<T>::from(right ? (double)m : -(double)m);
+        const bool face = right ? (rwR0 == cdx_ - x) : (rwR0 == x);
+        if (!face) {
+          x = right ? x + rwR0 : x - rwR0;                     // the interior end of the interval
+        } else {                                               // the cell face: crossing as :130-170
+          bool dead = false;
+          x = right ? cdx_ : zero;
+          if (mu > zero) {
+            if (ix == C - 1) {
+              if (g.bc[1] == IMC_REFLECT) mu = -mu; else dead = true;
+            }
+            if (!dead) { ix += 1; x = zero; }
+          }
+          if (!dead && mu < zero) {
+            if (ix == 0) {
+              if (g.bc[0] == IMC_REFLECT) mu = -mu; else dead = true;
+            } else {
+              ix -= 1; x = dx_of(ix);
+            }
+          }
+          if (dead) {
+            lbin = NB - 1;
+            lval = (Acc)(E / (W)g.scales[k]);
+            CK[slot] = DEAD;
+            has = false;
+          }
+        }
+      } else {                                                 // census inside the interval
+        const W Enew = E * (W)__expf(-sdf * s_rem);
+        val = (Acc)((E - Enew) * cidx_);
+        E = Enew;
+        const float y = rw_census_y(s_rem / spa, u24(q.r[2]), slot, ev, g.seed, g.step);
+        const float xn = fminf(fmaxf((float)x + R0 * y, 0.0f), (float)cdx_);
+        float m = 1.0f - 2.0f * u24(q.r[3]);
+        if (m == 0.0f) m = 0x1p-24f;
+        CK[slot] = (uint32_t)ix | ((uint32_t)k << KSHIFT);
+        X[slot] = Tm<T>::from((double)xn);
+        MU[slot] = Tm<T>::from((double)m);
+        TT[slot] = zero;
+        EE[slot] = E;
+        ++cens;
+        has = false;
+      }
+      ++ev;
+    } else if (SURR && want) {
       ++segs;
       ++nsurr;
       const u4 q = philox(slot, ev, TAG_SURR, 0u, g.seed, g.step);
@@ -1134,13 +1263,13 @@ __global__ __launch_bounds__(TPB) void k_transport(SoA s, Geo g, Scalars* sc, vo
 
   segs = wave_sum(segs);
   cens = wave_sum(cens);
-  if (SURR) nsurr = wave_sum(nsurr);
+  if (SURR || RW) nsurr = wave_sum(nsurr);
   ntrunc = (uint32_t)wave_sum((unsigned long long)ntrunc);
   if (lane == 0) {
     atomicAdd(&sc->segments, segs);
     atomicAdd(&sc->census_out, cens);
     if (ntrunc) atomicAdd(&sc->truncated, ntrunc);
-    if (SURR) atomicAdd(&sc->surr_events, nsurr);
+    if (SURR || RW) atomicAdd(&sc->surr_events, nsurr);
   }
   if (TM == TM_HASH) {                    // claimed slots go to the block's global copy once
     __syncthreads();
@@ -1177,6 +1306,7 @@ static void pick_tally(const Ctx& c, int nbins, int* tally_mode, size_t* lds_byt
 }
 
 static bool use_surrogate(const Ctx& c) { return c.surr_enabled && !c.is2d; }
+static bool use_random_walk(const Ctx& c) { return c.rw_enabled && !c.is2d && !use_surrogate(c); }
 
 // Bytes of the 1D LDS cell table, 0 when it is not used (2D, or more than CELL_LDS_MAX).
 static size_t cell_table_bytes(const Ctx& c) {
@@ -1186,32 +1316,38 @@ static size_t cell_table_bytes(const Ctx& c) {
   return b <= CELL_LDS_MAX ? b : 0;
 }
 
-// Dispatch over (precision, tally mode, geometry, surrogate, LDS cell table).  F(P, TM, D, S, L)
-// expands to a statement using the kernel k_transport<P, TM, D, S, L>.  The surrogate variants
-// exist for the block-row and global modes only.
-#define IMC_TM_DISPATCH(F, P, D, S, L)                                                   \
+// Dispatch over (precision, tally mode, geometry, acceleration, LDS cell table).  F(P, TM, D, A, L)
+// expands to a statement using the kernel k_transport<P, TM, D, A, L>; A is 0 (plain events),
+// 1 (learned surrogate) or 2 (analytic random walk).  The accelerated variants exist for the
+// block-row and global modes only.
+#define IMC_TM_DISPATCH(F, P, D, A, L)                                                   \
   do {                                                                                   \
-    if (tm == TM_WAVEROW) F(P, TM_WAVEROW, D, S, L);                                     \
-    else if (tm == TM_BLOCKROW) F(P, TM_BLOCKROW, D, S, L);                              \
-    else if (tm == TM_HASH) F(P, TM_HASH, D, S, L);                                      \
-    else F(P, TM_GLOBAL, D, S, L);                                                       \
+    if (tm == TM_WAVEROW) F(P, TM_WAVEROW, D, A, L);                                     \
+    else if (tm == TM_BLOCKROW) F(P, TM_BLOCKROW, D, A, L);                              \
+    else if (tm == TM_HASH) F(P, TM_HASH, D, A, L);                                      \
+    else F(P, TM_GLOBAL, D, A, L);                                                       \
   } while (0)
-#define IMC_TM_DISPATCH_SURR(F, P, L)                                                    \
+#define IMC_TM_DISPATCH_ACC(F, P, A, L)                                                  \
   do {                                                                                   \
-    if (tm == TM_BLOCKROW) F(P, TM_BLOCKROW, false, true, L);                            \
-    else F(P, TM_GLOBAL, false, true, L);                                                \
+    if (tm == TM_BLOCKROW) F(P, TM_BLOCKROW, false, A, L);                               \
+    else F(P, TM_GLOBAL, false, A, L);                                                   \
+  } while (0)
+#define IMC_TRANSPORT_DISPATCH_L(F, P, L)                                                \
+  do {                                                                                   \
+    if (surr_) IMC_TM_DISPATCH_ACC(F, P, 1, L);                                          \
+    else if (rw_) IMC_TM_DISPATCH_ACC(F, P, 2, L);                                       \
+    else IMC_TM_DISPATCH(F, P, false, 0, L);                                             \
   } while (0)
 #define IMC_TRANSPORT_DISPATCH_P(F, P)                                                   \
   do {                                                                                   \
-    if (c.is2d) IMC_TM_DISPATCH(F, P, true, false, false);                               \
-    else if (cell_) { if (surr_) IMC_TM_DISPATCH_SURR(F, P, true);                       \
-                      else IMC_TM_DISPATCH(F, P, false, false, true); }                  \
-    else { if (surr_) IMC_TM_DISPATCH_SURR(F, P, false);                                 \
-           else IMC_TM_DISPATCH(F, P, false, false, false); }                            \
+    if (c.is2d) IMC_TM_DISPATCH(F, P, true, 0, false);                                   \
+    else if (cell_) IMC_TRANSPORT_DISPATCH_L(F, P, true);                                \
+    else IMC_TRANSPORT_DISPATCH_L(F, P, false);                                          \
   } while (0)
 #define IMC_TRANSPORT_DISPATCH(F)                                                        \
   do {                                                                                   \
     const bool surr_ = use_surrogate(c);                                                 \
+    const bool rw_ = use_random_walk(c);                                                 \
     const bool cell_ = cell_table_bytes(c) > 0;                                          \
     if (c.prec == 64) IMC_TRANSPORT_DISPATCH_P(F, 64);                                   \
     else if (c.prec == 32) IMC_TRANSPORT_DISPATCH_P(F, 32);                              \
@@ -1220,7 +1356,7 @@ static size_t cell_table_bytes(const Ctx& c) {
 
 int transport_choose(const Ctx& c, int* grid_blocks, int* tally_mode, size_t* lds_bytes) {
   pick_tally(c, c.NB, tally_mode, lds_bytes);
-  if (use_surrogate(c) && (*tally_mode == TM_WAVEROW || *tally_mode == TM_HASH)) {
+  if ((use_surrogate(c) || use_random_walk(c)) && (*tally_mode == TM_WAVEROW || *tally_mode == TM_HASH)) {
     *tally_mode = TM_GLOBAL;
     *lds_bytes = 0;
   }
@@ -1253,6 +1389,8 @@ hipError_t launch_transport(Ctx& c, int grid, int tm, size_t lds) {
   sa.map = surr_map(c.surr_box);
   sa.tally_bytes = (int)lds;
   sa.cells = (const float4*)c.d_surr_cells;
+  sa.rw_table = c.d_rw_table;
+  sa.rw_min = (float)c.rw_min;
   if (use_surrogate(c)) {
     const unsigned gcell = (unsigned)((c.C + 255) / 256);
 #define SC(P) k_surr_cells<P><<<gcell, 256, 0, c.stream>>>(c.C, c.surr_box, (const Prec<P>::T*)c.d_dxs,            \

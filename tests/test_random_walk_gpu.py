@@ -1,0 +1,81 @@
+"""The analytic random walk (imc_random_walk_enable) on the GPU.
+
+* With a threshold no cell reaches, every lane takes the plain event loop: the random-walk
+  kernel then reproduces plain transport (FP64: identical populations and segment counts,
+  temperatures to 1e-12, the rounding floor of unordered tally atomics).
+* On the thick/thin block workload (12 steps, 4 seeds each) the random walk reproduces the
+  plain-transport temperature profile within 5 standard errors + 2 % per cell (the diffusion
+  limit is an approximation near the walk's threshold), with fewer event-loop passes, and
+  every step balances energy (the walk conserves energy exactly).
+"""
+import os
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _c4_inputs(n_input=200000, n_max=2000000):
+    from MixedPrecisionIMC import Input
+    inp = Input.readInputs(os.path.join(ROOT, "decks", "c4_thickthin_fp32.txt"))
+    inp["NINPUT"] = str(n_input)
+    inp["NMAX"] = str(n_max)
+    return inp
+
+
+def test_random_walk_without_eligible_cells_is_plain_transport():
+    import MixedPrecisionIMC as M
+    print("[random walk] r_min 1e9 vs plain, FP64", flush=True)
+    runs = []
+    for rw in (False, True):
+        inp = _c4_inputs(n_input=50000, n_max=500000)
+        mesh, sv, parts = M.setup(inp, precision=64, seed=31)
+        if rw:
+            parts.random_walk_enable(True, r_min=1e9)
+        for _ in range(3):
+            M.step(inp, mesh, sv, parts, "CONSTANT")
+        tm = parts.timing()
+        runs.append((mesh.temp.copy(), parts.n, tm["segments"], tm["surrogate_events"]))
+        parts.close()
+    (tp, n_p, sp, _), (tr, n_r, sr, er) = runs
+    assert n_p == n_r and sp == sr and er == 0
+    np.testing.assert_allclose(tr, tp, rtol=1e-12, atol=0)
+
+
+def test_random_walk_reproduces_plain_tallies():
+    import MixedPrecisionIMC as M
+    steps, seeds = 12, 4
+    plain, walk, seg_p, seg_w, ev_w = [], [], 0, 0, 0
+    for sd in range(seeds):
+        print(f"[random walk] thick/thin seed {sd}", flush=True)
+        for rw in (False, True):
+            inp = _c4_inputs()
+            mesh, sv, parts = M.setup(inp, seed=(900 if rw else 500) + sd)
+            if rw:
+                parts.random_walk_enable(True, r_min=5.0)
+            for _ in range(steps):
+                M.step(inp, mesh, sv, parts, "CONSTANT")
+            tm = parts.timing()
+            assert max(abs(e) for e in sv.energy_errors) < 2e-4
+            if rw:
+                seg_w += tm["segments"]; ev_w += tm["surrogate_events"]
+                walk.append(mesh.temp.copy())
+            else:
+                seg_p += tm["segments"]
+                plain.append(mesh.temp.copy())
+            parts.close()
+    plain, walk = np.array(plain), np.array(walk)
+    mp, mw = plain.mean(0), walk.mean(0)
+    se = np.sqrt(plain.var(0, ddof=1) / seeds + walk.var(0, ddof=1) / seeds)
+    ratio = np.abs(mp - mw) / (5 * se + 0.02 * np.abs(mp))
+    worst = int(np.argmax(ratio))
+    msg = (f"segments plain {seg_p} random walk {seg_w} (walk events {ev_w}); worst cell {worst}: "
+           f"plain {mp[worst]:.5f} walk {mw[worst]:.5f} se {se[worst]:.2e} ratio {ratio[worst]:.3f}; "
+           f"max rel diff {np.max(np.abs(mp - mw) / mp):.4f}")
+    print(msg)
+    assert ev_w > 0, msg
+    assert ratio[worst] < 1.0, msg
+    assert seg_w < seg_p, msg
