@@ -167,3 +167,36 @@ def test_abi_config_struct_layout():
     # imc_config as declared in include/imc_hip.h: 10 int32 + 2 int32 + 16 doubles + 3 doubles + int64 + 8 int32
     assert ctypes.sizeof(_hip.ImcConfig) == 4 * 12 + 8 * 19 + 8 + 4 * 8
     assert _hip.ImcConfig.energyscales.offset == 48
+
+
+def test_benchmark_comparison_and_saved_arrays(tmp_path):
+    """Output.plotting at t_end with BENCHMARK = TRUE and SAVEVARS = TRUE (the reference's Su-Olson
+    deck): the run's PLOTVARS field (radenergydens) is read at each XBENCH abscissa by linear
+    interpolation between cell centres, and the arrays of imc_output.jl:51-57 plus the table and
+    the interpolated values are saved.  A field that is linear in x is interpolated exactly."""
+    from MixedPrecisionIMC import Constants, Input, Mesh, Output
+    inp = Input.readInputs(os.path.join(ROOT, "tests", "golden", "SuOlson.txt"))
+    Constants.set_constants(inp)
+    mesh = Mesh.mesh_generation(inp)
+    mesh.radenergydens = 2.0 - 0.25 * mesh.centers
+    mesh.temp_saved = [mesh.temp.copy()]
+    mesh.radenergy_saved = [mesh.radenergydens.copy()]
+    mesh.matenergy_saved = [mesh.matenergydens.copy()]
+    cmp = Output.benchmark_comparison(inp, mesh)
+    xb, yb = np.array(inp["XBENCH"]), np.array(inp["YBENCH"])
+    inside = (xb >= mesh.centers[0]) & (xb <= mesh.centers[-1])
+    np.testing.assert_allclose(cmp["imc"][inside], 2.0 - 0.25 * xb[inside], rtol=1e-12)
+    np.testing.assert_allclose(cmp["diff"], cmp["imc"] - yb)
+    assert cmp["field"] == "radenergydens" and cmp["max_abs_diff"] == pytest.approx(np.max(np.abs(cmp["diff"])))
+    sv = M.SimVars(t=1.0, dt=0.01, dt0=0.0, k=0.0, dtmax=0.0, t_end=1.0, timesteps=[0.01, 1.0], iterations=0,
+                   n_input=1, n_max=1, cellmin=1.0, pairwise="FALSE", BC=(0, 0, 0, 0), precision=64, geometry="1D")
+    Output.plotting(inp, mesh, sv, outdir=str(tmp_path))
+    files = list(tmp_path.glob("*.npz"))
+    assert len(files) == 1
+    with np.load(files[0]) as z:
+        for key in ("timesteps", "centers", "temp_saved", "radenergy_saved", "matenergy_saved", "xbench", "ybench",
+                    "imc_at_xbench"):
+            assert key in z.files, key
+        np.testing.assert_array_equal(z["imc_at_xbench"], cmp["imc"])
+    inp["BENCHMARK"] = "FALSE"
+    assert Output.benchmark_comparison(inp, mesh) is None
