@@ -276,6 +276,7 @@ int imc_set_step_fields(imc_ctx* h, int64_t step, double dt, const double* sigma
                         const double* const* t_surf) {
   if (!h) return IMC_EINVAL;
   Ctx& c = h->c;
+  c.step_counters_clear = false;      // this call leaves the scalar counters in use
   if (!c.geo_set) return fail(c, IMC_EINVAL, "imc_set_geometry must be called first");
   if (!sigma_a || !sigma_s || !fleck || !temp || !radsource || !t_surf)
     return fail(c, IMC_EINVAL, "null step field");
@@ -329,6 +330,7 @@ int imc_source(imc_ctx* h, int64_t n_input, int64_t n_max, double cellmin, doubl
                double* totalenergy, int64_t* n_alive) {
   if (!h) return IMC_EINVAL;
   Ctx& c = h->c;
+  c.step_counters_clear = false;      // this call leaves the scalar counters in use
   if (!c.step_set) return fail(c, IMC_EINVAL, "imc_set_step_fields must be called before imc_source");
   if (n_input < 0 || n_max < 0 || !(cellmin >= 0.0)) return fail(c, IMC_EINVAL, "negative particle budget");
   HIPCHK(hipEventRecord(c.ev0, c.stream), "event");
@@ -384,6 +386,7 @@ int imc_source(imc_ctx* h, int64_t n_input, int64_t n_max, double cellmin, doubl
 int imc_transport(imc_ctx* h, double* energydep, double* lostenergy, int64_t* segments) {
   if (!h) return IMC_EINVAL;
   Ctx& c = h->c;
+  c.step_counters_clear = false;      // this call leaves the scalar counters in use
   if (!c.step_set) return fail(c, IMC_EINVAL, "imc_set_step_fields must be called before imc_transport");
   int grid, tm; size_t lds;
   transport_choose(c, &grid, &tm, &lds);
@@ -422,6 +425,7 @@ int imc_transport(imc_ctx* h, double* energydep, double* lostenergy, int64_t* se
 int imc_clean(imc_ctx* h, int64_t* n_alive) {
   if (!h) return IMC_EINVAL;
   Ctx& c = h->c;
+  c.step_counters_clear = false;      // this call leaves the scalar counters in use
   if (!c.step_set) return fail(c, IMC_EINVAL, "imc_set_step_fields must be called before imc_clean");
   HIPCHK(hipEventRecord(c.ev0, c.stream), "event");
   HIPCHK(launch_clean(c, c.h_sc.n_total), "clean kernels");
@@ -509,6 +513,7 @@ int imc_export_particles(imc_ctx* h, double* rec, int64_t cap, int64_t* nout) {
 int imc_import_particles(imc_ctx* h, const double* rec, int64_t n) {
   if (!h) return IMC_EINVAL;
   Ctx& c = h->c;
+  c.step_counters_clear = false;      // this call leaves the scalar counters in use
   if (n < 0 || (n > 0 && !rec)) return fail(c, IMC_EINVAL, "bad record buffer");
   int rc = grow(c, n, 0);
   if (rc) return rc;
@@ -617,8 +622,9 @@ int imc_set_material(imc_ctx* h, const imc_material* m) {
     ts = {m->t_surf[0][0], m->t_surf[1][0]};
   }
   if (!c.d_mat) {
-    if (dalloc(&c.d_mat, 8 * MAT_N * C) != hipSuccess) return fail(c, IMC_ENOMEM, "material allocation failed");
-    if (hipHostMalloc((void**)&c.h_out, 8 * 4 * C, hipHostMallocDefault) != hipSuccess) {
+    if (dalloc(&c.d_mat, 8 * MAT_N * C + sizeof(Scalars)) != hipSuccess)
+      return fail(c, IMC_ENOMEM, "material allocation failed");
+    if (hipHostMalloc((void**)&c.h_out, 8 * MAT_OUT * C + sizeof(Scalars), hipHostMallocDefault) != hipSuccess) {
       c.h_out = nullptr;
       return fail(c, IMC_ENOMEM, "pinned result allocation failed");
     }
@@ -633,6 +639,7 @@ int imc_set_material(imc_ctx* h, const imc_material* m) {
   HIPCHK(hipMemcpy(c.d_mat, st.data(), 8 * MAT_N * C, hipMemcpyHostToDevice), "material H2D");
   HIPCHK(hipMemcpy(c.d_radsrc, m->radsource, 8 * C, hipMemcpyHostToDevice), "radsource H2D");
   HIPCHK(hipMemcpy(c.d_tsurf, ts.data(), 8 * ts.size(), hipMemcpyHostToDevice), "t_surf H2D");
+  c.step_counters_clear = false;
   c.mat_linearized = m->linearized != 0;
   c.mat_marshak = m->marshak != 0;
   c.mat_alpha = m->alpha;
@@ -653,8 +660,12 @@ int imc_step(imc_ctx* h, int64_t step, double dt, int32_t first, int64_t n_input
   const size_t C = (size_t)c.C;
   // Update.update on the device, then the per-cell tables of the tracking type
   HIPCHK(hipEventRecord(c.evs[0], c.stream), "event");
-  HIPCHK(launch_zero(c, (char*)c.d_sc + offsetof(Scalars, prep_underflow),
-                     offsetof(Scalars, pad_) - offsetof(Scalars, prep_underflow)), "zero flags");
+  // the previous imc_step's last kernel leaves the flags and transport counters at zero
+  const bool clear = c.step_counters_clear;
+  c.step_counters_clear = false;
+  if (!clear)
+    HIPCHK(launch_zero(c, (char*)c.d_sc + offsetof(Scalars, prep_underflow),
+                       offsetof(Scalars, pad_) - offsetof(Scalars, prep_underflow)), "zero flags");
   HIPCHK(launch_update(c, first != 0), "update kernel");
   HIPCHK(launch_prep(c), "prep kernel");
   if (c.prec == 16) {                   // binary16 can round 1/sigma_c to 0: refuse before sourcing
@@ -706,8 +717,9 @@ int imc_step(imc_ctx* h, int64_t step, double dt, int32_t first, int64_t n_input
   // transport
   int grid, tm; size_t lds;
   transport_choose(c, &grid, &tm, &lds);
-  HIPCHK(launch_zero(c, (char*)c.d_sc + offsetof(Scalars, segments),
-                     offsetof(Scalars, prep_underflow) - offsetof(Scalars, segments)), "zero scalars");
+  if (!clear)
+    HIPCHK(launch_zero(c, (char*)c.d_sc + offsetof(Scalars, segments),
+                       offsetof(Scalars, prep_underflow) - offsetof(Scalars, segments)), "zero scalars");
   if (tm == TM_GLOBAL || tm == TM_HASH)
     HIPCHK(launch_zero(c, c.d_gtally, 8 * (size_t)c.NB * gtally_copies(c.NB)), "zero tallies");
   HIPCHK(hipEventRecord(c.evk0, c.stream), "event");
@@ -722,14 +734,13 @@ int imc_step(imc_ctx* h, int64_t step, double dt, int32_t first, int64_t n_input
   HIPCHK(hipEventRecord(c.evs[2], c.stream), "event");
   HIPCHK(launch_temperature(c), "temperature kernel");
   HIPCHK(launch_step_sums(c), "energy sums kernel");
-  const int out_slot[4] = {MAT_TEMP, -1, MAT_E, MAT_INC};
-  for (int q = 0; q < 4; ++q) {
-    const double* from = out_slot[q] < 0 ? c.d_rad : c.d_mat + out_slot[q] * C;
-    HIPCHK(hipMemcpyAsync(c.h_out + q * C, from, 8 * C, hipMemcpyDeviceToHost, c.stream), "results D2H");
-  }
-  HIPCHK(hipMemcpyAsync(&c.h_sc, c.d_sc, sizeof(Scalars), hipMemcpyDeviceToHost, c.stream), "scalars D2H");
+  // temperature, radenergydens, matenergydens, nrg_inc and the scalar snapshot: one copy
+  HIPCHK(hipMemcpyAsync(c.h_out, c.d_mat + (size_t)MAT_TEMP * C, 8 * MAT_OUT * C + sizeof(Scalars),
+                        hipMemcpyDeviceToHost, c.stream), "results D2H");
   HIPCHK(hipEventRecord(c.evs[3], c.stream), "event");
   HIPCHK(hipStreamSynchronize(c.stream), "step sync");
+  std::memcpy(&c.h_sc, c.h_out + MAT_OUT * C, sizeof(Scalars));
+  c.step_counters_clear = true;
   c.rad_valid = false;
   c.timing.ms_source += elapsed(c.evs[0], c.ev1);
   c.timing.ms_transport += elapsed(c.ev1, c.evs[1]);

@@ -167,8 +167,9 @@ struct Ctx {
   bool mat_set = false;
   int mat_linearized = 0, mat_marshak = 0;
   double mat_alpha = 0.0;
-  double* d_mat = nullptr;         // [sa_coef, sa_pow, ss_coef, ss_pow, bee, temp, matE, nrg_inc] x C
-  double* h_out = nullptr;         // pinned: temp, radenergydens, matenergydens, nrg_inc (C each)
+  double* d_mat = nullptr;         // MAT_N x C (MatSlot), then a Scalars snapshot of the step
+  double* h_out = nullptr;         // pinned copy of MAT_TEMP .. MAT_INC and the snapshot
+  bool step_counters_clear = false;  // the last imc_step cleared the counters the next one needs at 0
 
   // timing
   hipEvent_t ev0 = nullptr, ev1 = nullptr, evk0 = nullptr, evk1 = nullptr;
@@ -177,7 +178,10 @@ struct Ctx {
 };
 
 // Material state offsets inside Ctx::d_mat (multiples of C).
-enum MatSlot { MAT_SA_COEF = 0, MAT_SA_POW, MAT_SS_COEF, MAT_SS_POW, MAT_BEE, MAT_TEMP, MAT_E, MAT_INC, MAT_N };
+// MAT_TEMP .. MAT_INC are the step's results, in the order imc_step returns them, followed in
+// the same allocation by the Scalars snapshot: one copy brings them all back.
+enum MatSlot { MAT_SA_COEF = 0, MAT_SA_POW, MAT_SS_COEF, MAT_SS_POW, MAT_BEE, MAT_TEMP, MAT_RAD, MAT_E, MAT_INC, MAT_N };
+constexpr int MAT_OUT = MAT_N - MAT_TEMP;     // result arrays per step
 
 Geo make_geo(const Ctx& c);
 int state_bytes(int prec, bool is2d);

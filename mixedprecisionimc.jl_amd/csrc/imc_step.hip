@@ -96,18 +96,21 @@ hipError_t launch_update(Ctx& c, int first) {
 }
 
 // nrg_inc = sum_k (energydep_k - emitted_k) / scale_k; matE += nrg_inc; T = matE^(1/4) when
-// linearized, else T += nrg_inc / bee (imc_tally.jl:46-75).
+// linearized, else T += nrg_inc / bee (imc_tally.jl:46-75).  Also stages radenergydens next
+// to the other results.
 __global__ __launch_bounds__(256) void k_temperature(int C, int K, Geo g, const double* __restrict__ dep,
                                                      const double* __restrict__ emitted, const double* __restrict__ bee,
-                                                     double* __restrict__ temp, double* __restrict__ matE,
-                                                     double* __restrict__ inc_out, int linearized) {
+                                                     const double* __restrict__ rad, double* __restrict__ mat,
+                                                     int linearized) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= C) return;
   double inc = 0.0;
   for (int k = 0; k < K; ++k) inc = inc + (dep[(size_t)k * C + i] - emitted[(size_t)k * C + i]) / g.scales[k];
-  const double m = matE[i] + inc;
-  matE[i] = m;
-  inc_out[i] = inc;
+  double* temp = mat + (size_t)MAT_TEMP * C;
+  const double m = mat[(size_t)MAT_E * C + i] + inc;
+  mat[(size_t)MAT_E * C + i] = m;
+  mat[(size_t)MAT_INC * C + i] = inc;
+  mat[(size_t)MAT_RAD * C + i] = rad[i];
   temp[i] = linearized ? pow(m, 0.25) : temp[i] + inc / bee[i];
 }
 
@@ -115,8 +118,7 @@ hipError_t launch_temperature(Ctx& c) {
   const size_t C = (size_t)c.C;
   const Geo g = make_geo(c);
   k_temperature<<<(unsigned)((C + 255) / 256), 256, 0, c.stream>>>(
-      c.C, c.K, g, c.d_dep, c.d_emitted, c.d_mat + MAT_BEE * C, c.d_mat + MAT_TEMP * C, c.d_mat + MAT_E * C,
-      c.d_mat + MAT_INC * C, c.mat_linearized);
+      c.C, c.K, g, c.d_dep, c.d_emitted, c.d_mat + MAT_BEE * C, c.d_rad, c.d_mat, c.mat_linearized);
   return hipGetLastError();
 }
 
@@ -125,7 +127,7 @@ hipError_t launch_temperature(Ctx& c) {
 // bin.  Strided per-thread partials and a fixed tree: the result does not depend on timing.
 __global__ __launch_bounds__(1024) void k_step_sums(int C, int K, int nx, Geo g, const double* __restrict__ dx,
                                                     const double* __restrict__ dy, const double* __restrict__ dep,
-                                                    const double* __restrict__ rad, Scalars* sc) {
+                                                    const double* __restrict__ rad, Scalars* sc, Scalars* snap) {
   __shared__ double part[1024];
   const int tid = threadIdx.x;
   double edep = 0.0;
@@ -148,12 +150,17 @@ __global__ __launch_bounds__(1024) void k_step_sums(int C, int K, int nx, Geo g,
   if (tid == 0) {
     sc->E_dep = edep;
     sc->E_lost = dep[(size_t)K * C];
+    *snap = *sc;               // the step's scalars, next to its arrays for one copy back
+    // and the counters the next step needs at zero (no fill launches there)
+    sc->segments = 0; sc->census_out = 0; sc->surr_events = 0; sc->work = 0; sc->truncated = 0;
+    sc->prep_underflow = 0; sc->field_error = 0;
   }
 }
 
 hipError_t launch_step_sums(Ctx& c) {
   const Geo g = make_geo(c);
-  k_step_sums<<<1, 1024, 0, c.stream>>>(c.C, c.K, c.nx, g, c.d_dx, c.d_dy, c.d_dep, c.d_rad, c.d_sc);
+  k_step_sums<<<1, 1024, 0, c.stream>>>(c.C, c.K, c.nx, g, c.d_dx, c.d_dy, c.d_dep, c.d_rad, c.d_sc,
+                                        (Scalars*)(c.d_mat + (size_t)MAT_N * c.C));
   return hipGetLastError();
 }
 
