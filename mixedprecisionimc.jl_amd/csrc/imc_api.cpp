@@ -123,6 +123,21 @@ int state_bytes(int prec, bool is2d) {
   return 4 + (is2d ? 4 : 3) * T + 2 * W;
 }
 
+// Bytes of per-block tally partials: (sum, compensation) rows per block for the tally modes
+// that flush through them (per-wave rows, block row); the global and hashed modes add into
+// global bins and need none.
+size_t partials_needed(const Ctx& c) {
+  int gt, tmt, gc, tmc; size_t lt, lc;
+  transport_choose(c, &gt, &tmt, &lt);
+  clean_choose(c, &gc, &tmc, &lc);
+  const size_t acc = c.prec == 64 ? 8 : 4;
+  const auto rows = [](int tm) { return tm == TM_WAVEROW || tm == TM_BLOCKROW; };
+  size_t need = 256;
+  if (rows(tmt)) need = std::max(need, (size_t)gt * c.NB * 2 * acc);
+  if (rows(tmc)) need = std::max(need, (size_t)gc * c.C * 2 * acc);
+  return need;
+}
+
 Geo make_geo(const Ctx& c) {
   Geo g;
   g.nx = c.nx; g.ny = c.ny; g.C = c.C; g.K = c.K; g.NB = c.NB;
@@ -213,12 +228,7 @@ int imc_create(imc_ctx** out, const imc_config* cfg) {
   ok &= dalloc(&c.d_rad, 8 * C) == hipSuccess;
   ok &= dalloc(&c.d_sc, sizeof(Scalars)) == hipSuccess;
   if (!ok) return fail(c, IMC_ENOMEM, "device allocation failed");
-  // tally partials: sized for the larger of transport (NB bins) and clean (C bins)
-  int gt, tmt, gc, tmc; size_t lt, lc;
-  transport_choose(c, &gt, &tmt, &lt);
-  clean_choose(c, &gc, &tmc, &lc);
-  const size_t acc = c.prec == 64 ? 8 : 4;
-  c.partial_bytes = std::max((size_t)gt * c.NB * 2 * acc, (size_t)gc * C * 2 * acc);
+  c.partial_bytes = partials_needed(c);
   if (dalloc(&c.d_partial, c.partial_bytes) != hipSuccess) return fail(c, IMC_ENOMEM, "tally partial allocation failed");
   const int64_t cap = std::max<int64_t>(cfg->capacity, 4096);
   if (alloc_soa(c, c.buf[0], cap) != hipSuccess || alloc_soa(c, c.buf[1], cap) != hipSuccess)
@@ -1046,10 +1056,7 @@ int imc_random_walk_enable(imc_ctx* h, int32_t on, double r_min) {
 
 // The tally partial buffer must cover the transport grid of the active configuration.
 static int fit_partials(Ctx& c) {
-  int grid, tm; size_t lds;
-  transport_choose(c, &grid, &tm, &lds);
-  const size_t acc = c.prec == 64 ? 8 : 4;
-  const size_t need = (size_t)grid * c.NB * 2 * acc;
+  const size_t need = partials_needed(c);
   if (need > c.partial_bytes) {
     (void)hipFree(c.d_partial);
     c.d_partial = nullptr;

@@ -675,7 +675,7 @@ __global__ __launch_bounds__(256) void k_reduce_partials(const Acc* __restrict__
 }
 
 hipError_t launch_reduce_partials(Ctx& c, int nblocks, int tally_mode, int nbins, double* out) {
-  if (tally_mode == TM_GLOBAL) return hipSuccess;
+  if (tally_mode == TM_GLOBAL || tally_mode == TM_HASH) return hipSuccess;   // no per-block partials
   if (c.prec == 64)
     k_reduce_partials<double><<<nbins, 256, 0, c.stream>>>((const double*)c.d_partial, nblocks, nbins, out);
   else
