@@ -224,6 +224,47 @@ def surrogate_leg(steps=20, n_input=400000, seeds=4):
 
 
 DECK_CHANNEL = os.path.join(ROOT, "decks", "c3_channel_fp16.txt")
+DECK_LARGE2D = os.path.join(ROOT, "decks", "c5_blocks2d_fp16.txt")
+
+
+def large_2d_leg(warmup: int = 4, steps: int = 8):
+    """The large 2D workload (1024 x 1024 cells, 64 x 64 random material blocks, NINPUT 2e7,
+    NMAX 2e8; decks/c5_blocks2d_fp16.txt) on a window of its first steps.  Its deck asks for
+    FP16 tracking, which the engine refuses (the cold T^-3 blocks have 1/sigma_c below binary16's
+    range); the rate is measured with FP32 tracking and compensated FP32 tallies, each step one
+    device-resident imc_step."""
+    import MixedPrecisionIMC as M
+    from MixedPrecisionIMC import Input
+    from MixedPrecisionIMC._hip import ImcError
+
+    inp = Input.readInputs(DECK_LARGE2D)
+    ts = str(inp["TIMESTEPPING"]).upper()
+    out = {"workload": f"large 2D blocks (decks/c5_blocks2d_fp16.txt), steps {warmup}..{warmup + steps - 1}"}
+    mesh, simvars, particles = M.setup(inp, precision=16)
+    try:
+        M.step_device(inp, mesh, simvars, particles, ts)
+        out["fp16"] = "ran"
+    except ImcError as exc:
+        out["fp16"] = f"refused: {exc}"
+    particles.close()
+    mesh, simvars, particles = M.setup(inp, precision=32)
+    for _ in range(warmup):
+        M.step_device(inp, mesh, simvars, particles, ts)
+    particles.synchronize()
+    particles.reset_timing()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        M.step_device(inp, mesh, simvars, particles, ts)
+    particles.synchronize()
+    wall = time.perf_counter() - t0
+    tm = particles.timing()
+    out["fp32"] = {"histories_per_s": tm["histories"] / wall, "hot_path_histories_per_s": tm["histories"] / _hot_s(tm),
+                   "segments_per_s_transport_kernel": tm["segments"] / (tm["ms_transport_kernel"] / 1e3),
+                   "ms_per_step": 1e3 * wall / steps, "histories_per_step": tm["histories"] / steps,
+                   "tally_mode": TALLY_MODES.get(tm["transport_tally_mode"]),
+                   "max_abs_energy_error": max(abs(e) for e in simvars.energy_errors)}
+    particles.close()
+    return out
 
 
 def precision_study(steps=50, seeds=4, n_input=None):
@@ -287,6 +328,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-surrogate", action="store_true")
     ap.add_argument("--no-host-update", action="store_true", help="skip the host-update comparison loop")
+    ap.add_argument("--no-large-2d", action="store_true", help="skip the large 2D (1024 x 1024) leg")
     ap.add_argument("--blocks-per-cu", type=int, default=0, help="transport grid bound (0: automatic)")
     ap.add_argument("--precision-study", action="store_true",
                     help="also run the FP16/FP32/FP64 tally-error study on the 2D channel deck")
@@ -370,6 +412,10 @@ def main():
         line["surrogate"] = surrogate_leg()
         print(f"[bench] surrogate leg: FOM speedup {line['surrogate']['fom_speedup']:.2f}", file=sys.stderr,
               flush=True)
+    if not args.no_large_2d:
+        line["large_2d"] = large_2d_leg()
+        print(f"[bench] large 2D leg: {line['large_2d']['fp32']['histories_per_s']:.4g} histories/s at FP32",
+              file=sys.stderr, flush=True)
     if args.precision_study:
         line["precision_study"] = precision_study()
     print(json.dumps(line))

@@ -209,6 +209,18 @@ int imc_surrogate_enable(imc_ctx* ctx, int32_t on);
  * quirk Q6).  Enabling it disables the surrogate and the other way round.  1D only. */
 int imc_random_walk_enable(imc_ctx* ctx, int32_t on, double r_min);
 
+/* ---- Planck frequency sampling -----------------------------------------------------
+ * Sourcing.sample_planck (imc_sourcing.jl:372-399; unused by the reference's gray births,
+ * which set nu = 1 and leave `T * sample_planck()` commented out at imc_sourcing.jl:171,186,
+ * 209,232,342,362).  freq[i] = temp[i] * x_i with x_i drawn from the
+ * normalised Planck spectrum (15 / pi^4) x^3 / (e^x - 1), by the same mixture of Gamma(4, 1/n)
+ * laws with weights 90 / (pi^4 n^4); computed in FP64 on the GPU, every draw ends (the
+ * reference's loop does not end for a first uniform above 90 sum(n^-4) / pi^4 with the sum
+ * rounded to Float32: about one draw in 1e6).  Draw i is keyed by (the context's seed,
+ * `stream`, i): the same triple gives the same frequency.  Host arrays of n values; n = 0 is a
+ * no-op; every temp must be finite and >= 0 (IMC_EINVAL otherwise). */
+int imc_sample_planck(imc_ctx* ctx, int64_t n, const double* temp, uint32_t stream, double* freq);
+
 #ifdef __cplusplus
 }
 #endif

@@ -28,3 +28,18 @@ def sourcing(mesh, simvars, particles: Engine) -> None:
     K = len(mesh.energyscales)
     mesh.emittedenergy = emitted.reshape(mesh.shape + (K,), order="F")
     mesh.totalenergy = etot
+
+
+_DTYPES = {16: np.float16, 32: np.float32, 64: np.float64}
+
+
+def sample_planck(simvars, particles: Engine, temp=1.0, stream: int = 0) -> np.ndarray:
+    """Sourcing.sample_planck (imc_sourcing.jl:372-399): frequencies from the Planck spectrum,
+    returned in the run's PRECISION as the reference returns them.  `temp` (scalar or array)
+    multiplies the dimensionless sample, as at the reference's commented-out call site
+    (`mesh.temp[cellindex] * sample_planck()`, imc_sourcing.jl:209,232,342,362; surfaces use
+    `T_surface * sample_planck()`, :171,186); one frequency per entry of
+    `temp`.  Drawn on the GPU (imc_sample_planck), keyed by (seed, stream, index)."""
+    t = np.atleast_1d(np.asarray(temp, dtype=np.float64))
+    return particles.sample_planck(t, stream=stream).astype(_DTYPES[int(simvars.precision)]).reshape(t.shape)
+

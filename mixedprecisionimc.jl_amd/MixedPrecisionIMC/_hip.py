@@ -139,6 +139,7 @@ SYMBOLS = {
                                               ctypes.POINTER(ctypes.c_float)]),
     "imc_surrogate_enable": (ctypes.c_int, [_VP, ctypes.c_int32]),
     "imc_random_walk_enable": (ctypes.c_int, [_VP, ctypes.c_int32, ctypes.c_double]),
+    "imc_sample_planck": (ctypes.c_int, [_VP, ctypes.c_int64, _DP, ctypes.c_uint32, _DP]),
 }
 
 _FP = ctypes.POINTER(ctypes.c_float)
@@ -402,6 +403,14 @@ class Engine:
         self.surrogate_on = bool(on)
         if on:
             self.random_walk_on = False
+
+    def sample_planck(self, temp, stream: int = 0) -> np.ndarray:
+        """imc_sample_planck: frequencies temp_i * x_i, x_i from the normalised Planck
+        spectrum, keyed by (seed, stream, i)."""
+        t = np.ascontiguousarray(temp, dtype=np.float64).ravel()
+        out = np.empty_like(t)
+        self._check(lib().imc_sample_planck(self._h, t.size, _dp(t), int(stream) & 0xFFFFFFFF, _dp(out)))
+        return out
 
     def random_walk_enable(self, on: bool = True, r_min: float = 5.0):
         """imc_random_walk_enable: the analytic random walk in cells at least r_min collision

@@ -1023,6 +1023,29 @@ static double rw_exit_cdf(double a) {
 
 // Transport.MC_RW (imc_transport.jl:212-479) with the random-walk table of :786-797, as a
 // correct analytic walk (see the RW note in imc_kernels.hip).
+int imc_sample_planck(imc_ctx* h, int64_t n, const double* temp, uint32_t stream, double* freq) {
+  if (!h) return IMC_EINVAL;
+  Ctx& c = h->c;
+  if (n < 0 || n > (int64_t)INT32_MAX * 256) return fail(c, IMC_EINVAL, "sample count out of range");
+  if (n == 0) return IMC_OK;
+  if (!temp || !freq) return fail(c, IMC_EINVAL, "null temperature or frequency array");
+  for (int64_t i = 0; i < n; ++i)
+    if (!(std::isfinite(temp[i]) && temp[i] >= 0.0)) return fail(c, IMC_EINVAL, "temperatures must be finite and >= 0");
+  double *dt = nullptr, *df = nullptr;
+  if (dalloc(&dt, 8 * (size_t)n) != hipSuccess || dalloc(&df, 8 * (size_t)n) != hipSuccess) {
+    (void)hipFree(dt);
+    return fail(c, IMC_ENOMEM, "Planck sample buffers");
+  }
+  hipError_t e = hipMemcpyAsync(dt, temp, 8 * (size_t)n, hipMemcpyHostToDevice, c.stream);
+  if (e == hipSuccess) e = launch_planck(c, n, dt, df, stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(freq, df, 8 * (size_t)n, hipMemcpyDeviceToHost, c.stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c.stream);
+  (void)hipFree(dt);
+  (void)hipFree(df);
+  if (e != hipSuccess) return hipfail(c, e, "Planck sampling");
+  return IMC_OK;
+}
+
 int imc_random_walk_enable(imc_ctx* h, int32_t on, double r_min) {
   if (!h) return IMC_EINVAL;
   Ctx& c = h->c;

@@ -204,6 +204,7 @@ int clean_choose(const Ctx& c, int* grid_blocks, int* tally_mode, size_t* lds_by
 // Device-resident step (imc_step.hip).
 hipError_t launch_update(Ctx& c, int first);
 hipError_t launch_temperature(Ctx& c);
+hipError_t launch_planck(Ctx& c, int64_t n, const double* d_temp, double* d_freq, uint32_t stream);
 hipError_t launch_step_sums(Ctx& c);
 hipError_t surr_generate(Ctx& c, int64_t n, uint32_t seed, unsigned long long* d_events);
 hipError_t surr_make_weights(Ctx& c);

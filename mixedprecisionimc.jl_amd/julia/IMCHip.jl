@@ -261,4 +261,19 @@ function step!(e::Engine, mesh, simvars)
     return r[].radenergy
 end
 
+"""
+    sample_planck(e::Engine, temps::AbstractVector; stream = 0)
+
+Planck frequencies `temps[i] * x_i` (`Sourcing.sample_planck`, imc_sourcing.jl:372-399, with
+the temperature factor of its commented-out call site), drawn on the GPU in Float64 and keyed
+by (seed, stream, i).
+"""
+function sample_planck(e::Engine, temps::AbstractVector; stream::Integer = 0)
+    t = Vector{Float64}(temps)
+    out = similar(t)
+    check(e, ccall((:imc_sample_planck, LIB), Cint, (Ptr{Cvoid}, Int64, Ptr{Float64}, UInt32, Ptr{Float64}),
+                   e.ptr, length(t), t, UInt32(stream & 0xffffffff), out))
+    return out
+end
+
 end # module

@@ -858,4 +858,59 @@ int orc_run(const orc_run_cfg* cfg, orc_run_out* out) {
   return 0;
 }
 
+// Planck frequency sampling (imc_sourcing.jl:372-399, the Fleck-Cummings mixture of
+// Gamma(4, 1/n) laws with weights 90 / (pi^4 n^4)), restated with the engine's Philox layout:
+// sample i uses blocks (i, TAG_PLANCK = 6, j) keyed (seed, stream); n by FP64 inversion for
+// n <= 64, beyond that by rejection from the continuous x^-4 law (n = ceil(64 / cbrt(1 - v)),
+// accepted with probability 3 n^-4 / ((n-1)^-3 - n^-3)); x = -(sum of four logs) / n.
+// u_force >= 0 replaces the term uniform (to exercise the tail in tests).
+static double planck_term(int64_t i, uint32_t seed, uint32_t stream, double u_force, U4* q0o, U4* q1o, U4* q2o) {
+  constexpr uint32_t TAG_PLANCK = 6u;
+  const uint32_t i0 = (uint32_t)i, i1 = (uint32_t)((uint64_t)i >> 32);
+  const U4 q0 = philox(i0, i1, TAG_PLANCK, 0u, seed, stream);
+  const U4 q1 = philox(i0, i1, TAG_PLANCK, 1u, seed, stream);
+  const U4 q2 = philox(i0, i1, TAG_PLANCK, 2u, seed, stream);
+  *q0o = q0; *q1o = q1; *q2o = q2;
+  const double u = u_force >= 0.0 ? u_force : u53(q0.r[0], q0.r[1]);
+  const double w = 90.0 / (PI_D * PI_D * PI_D * PI_D);
+  double cum = 0.0, m = 0.0;
+  for (int k = 1; k <= 64; ++k) {
+    const double dk = (double)k;
+    cum += w / (dk * dk * dk * dk);
+    if (u < cum) { m = dk; break; }
+  }
+  if (m == 0.0) {
+    double v = u53(q2.r[2], q2.r[3]);
+    for (int t = 0; t < 64; ++t) {
+      m = std::ceil(64.0 / std::cbrt(1.0 - v));
+      if (m <= 64.0) m = 65.0;
+      const double a = 1.0 / (m - 1.0), b = 1.0 / m;
+      const double accept = 3.0 * (b * b * b * b) / (a * a * a - b * b * b);
+      const U4 qt = philox(i0, i1, TAG_PLANCK, 3u + (uint32_t)t, seed, stream);
+      if (u53(qt.r[0], qt.r[1]) < accept) break;
+      v = u53(qt.r[2], qt.r[3]);
+    }
+  }
+  return m;
+}
+
+void orc_planck(int64_t n, const double* temp, double* freq, uint32_t seed, uint32_t stream) {
+  const auto open0 = [](uint32_t hi, uint32_t lo) { return u53(hi, lo) + 0x1p-53; };
+  for (int64_t i = 0; i < n; ++i) {
+    U4 q0, q1, q2;
+    const double m = planck_term(i, seed, stream, -1.0, &q0, &q1, &q2);
+    const double s = std::log(open0(q0.r[2], q0.r[3])) + std::log(open0(q1.r[0], q1.r[1])) +
+                     std::log(open0(q1.r[2], q1.r[3])) + std::log(open0(q2.r[0], q2.r[1]));
+    freq[i] = temp[i] * (-s / m);
+  }
+}
+
+// The mixture term n of samples 0 .. n-1 (u_force < 0: the sampler's own uniform).
+void orc_planck_terms(int64_t n, uint32_t seed, uint32_t stream, double u_force, int64_t* terms) {
+  for (int64_t i = 0; i < n; ++i) {
+    U4 q0, q1, q2;
+    terms[i] = (int64_t)planck_term(i, seed, stream, u_force, &q0, &q1, &q2);
+  }
+}
+
 }  // extern "C"

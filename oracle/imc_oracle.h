@@ -102,6 +102,11 @@ typedef struct {
 
 int orc_run(const orc_run_cfg* cfg, orc_run_out* out);
 
+// Planck frequencies nu_i = temp_i * x_i, x from the normalised Planck spectrum (imc_sourcing.jl:372-399).
+void orc_planck(int64_t n, const double* temp, double* freq, uint32_t seed, uint32_t stream);
+// Mixture terms n of samples 0 .. n-1; u_force >= 0 replaces the term uniform.
+void orc_planck_terms(int64_t n, uint32_t seed, uint32_t stream, double u_force, int64_t* terms);
+
 #ifdef __cplusplus
 }
 #endif

@@ -82,12 +82,75 @@ def lib():
         L.orc_census_tally.argtypes = [ctypes.POINTER(Problem), ctypes.POINTER(Step), _DP, ctypes.c_int64, _DP]
         L.orc_run.restype = ctypes.c_int
         L.orc_run.argtypes = [ctypes.POINTER(RunCfg), ctypes.POINTER(RunOut)]
+        L.orc_planck.restype = None
+        L.orc_planck.argtypes = [ctypes.c_int64, _DP, _DP, ctypes.c_uint32, ctypes.c_uint32]
+        L.orc_planck_terms.restype = None
+        L.orc_planck_terms.argtypes = [ctypes.c_int64, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_double,
+                                       ctypes.POINTER(ctypes.c_int64)]
         _lib = L
     return _lib
 
 
 def _p(a):
     return a.ctypes.data_as(_DP) if a is not None else None
+
+
+def planck(temp, seed: int, stream: int) -> np.ndarray:
+    """Planck frequencies temp_i * x_i with the engine's Philox layout (imc_oracle.cpp:
+    orc_planck; the method of imc_sourcing.jl:372-399)."""
+    t = np.ascontiguousarray(temp, dtype=np.float64)
+    out = np.empty_like(t)
+    lib().orc_planck(t.size, _p(t), _p(out), seed, stream)
+    return out
+
+
+def planck_terms(n: int, seed: int, stream: int, u_force: float = -1.0) -> np.ndarray:
+    """Mixture terms n of samples 0 .. n-1 of `planck` (u_force >= 0 replaces the term uniform)."""
+    out = np.empty(n, dtype=np.int64)
+    lib().orc_planck_terms(n, seed, stream, u_force, out.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)))
+    return out
+
+
+def sample_planck_reference(uniforms, precision=np.float64, max_terms=1 << 20):
+    """Sourcing.sample_planck (imc_sourcing.jl:372-399) as written, for one draw: `uniforms`
+    yields rand(precision) values in the order the reference takes them (rn1, then rn1..rn4
+    after the term is chosen).  n and the running sum of n^-4 are kept in `precision`; the
+    test rn1 <= 90 nsum / pi^4 is formed in Float64 (the literal 90.0 and pi^4 promote).
+    Returns (x, n), or (None, n) when `max_terms` terms pass without the test succeeding --
+    the reference's loop has no other exit."""
+    one = precision(1.0)
+    n = one
+    nsum = one
+    rn1 = float(next(uniforms))
+    for _ in range(max_terms):
+        if rn1 <= 90.0 * float(nsum) / np.pi ** 4:
+            r = [precision(next(uniforms)) for _ in range(4)]
+            prod = precision(precision(precision(r[0] * r[1]) * r[2]) * r[3])
+            x = precision(-1.0 * float(np.log(prod)) / float(n))
+            return x, float(n)
+        n = precision(n + one)
+        nsum = precision(nsum + _inv_n4(n, precision))
+    return None, float(n)
+
+
+def _inv_n4(n, precision):
+    """precision(1.0 / n^4) with n^4 rounded to `precision` (Float16 n^4 overflows to Inf
+    from n = 16, so the term is 0 there)."""
+    with np.errstate(over="ignore"):
+        n4 = precision(float(n) ** 4)
+    return precision(1.0 / float(n4))
+
+
+def planck_nsum_limit(precision) -> float:
+    """The value at which the reference's running sum of n^-4 (kept in `precision`) stops
+    growing; 90 * limit / pi^4 is the largest first uniform its loop can accept."""
+    one = precision(1.0)
+    n, nsum = one, one
+    while True:
+        nxt = precision(nsum + _inv_n4(precision(n + one), precision))
+        if nxt == nsum:
+            return float(nsum)
+        n, nsum = precision(n + one), nxt
 
 
 def philox(ctr: Sequence[int], key: Sequence[int]):
