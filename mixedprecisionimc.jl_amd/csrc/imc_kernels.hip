@@ -889,6 +889,20 @@ __global__ __launch_bounds__(TPB) void k_transport(SoA s, Geo g, Scalars* sc, vo
   bool done = false;
   unsigned long long segs = 0, cens = 0, nsurr = 0;
   uint32_t ntrunc = 0;
+  // Without the LDS cell table, the coefficients of the lane's current cell stay in registers
+  // and are reloaded only when the lane enters another cell: a particle in an optically thick
+  // cell takes many events per visit, and each reload is a scattered load from a mesh-sized
+  // array.  The values are the same; only the loads are skipped.
+  int ccell = -1;
+  T c_dx = zero, c_is = zero, c_dy = zero;
+  W c_sd = (W)0, c_idx = (W)0, c_idy = (W)0;
+  // Each lane keeps the deposits to its current bin in an FP64 register sum and hands it to the
+  // tally only when it deposits into another bin (and once after the loop).  A particle in an
+  // optically thick cell deposits into the same bin for many events per visit; the tally then
+  // sees one add per visit instead of one per event (on large meshes each add is a scattered
+  // global atomic).  The sum is rounded once to the tally type per visit.
+  int abin = -1;
+  double aval = 0.0;
 
   for (;;) {
     // ---- lane refill from the wave's chunk ----
@@ -1139,13 +1153,18 @@ __global__ __launch_bounds__(TPB) void k_transport(SoA s, Geo g, Scalars* sc, vo
       ++segs;
       const u4 q = philox(slot, ev, TAG_EVENT, 0u, g.seed, g.step);
       const int cell = TWO_D ? ix + nx * iy : ix;
-      T cdx, cis;
-      W csd, cidx;
+      T cdx, cis, cdy = zero;
+      W csd, cidx, cidy = (W)1;
       if (CELL) {
         const CellRec<P> r = s_cell[ix];
         cdx = r.dxs; cis = r.isig; csd = r.sigdep; cidx = r.idxw;
       } else {
-        cdx = dxs[ix]; cis = isig[cell]; csd = sigdep[cell]; cidx = idxw[ix];
+        if (cell != ccell) {
+          ccell = cell;
+          c_dx = dxs[ix]; c_is = isig[cell]; c_sd = sigdep[cell]; c_idx = idxw[ix];
+          if (TWO_D) { c_dy = dys[iy]; c_idy = idyw[iy]; }
+        }
+        cdx = c_dx; cis = c_is; csd = c_sd; cidx = c_idx; cdy = c_dy; cidy = c_idy;
       }
       T db, dbx = zero, dby = zero;
       if (!TWO_D) {
@@ -1154,7 +1173,7 @@ __global__ __launch_bounds__(TPB) void k_transport(SoA s, Geo g, Scalars* sc, vo
         db = (mu > zero) ? qb : Tm<T>::abs_(qb);
       } else {
         dbx = Tm<T>::abs_(((ox > zero) ? (cdx - x) : x) / ox);                    // :538-542
-        dby = Tm<T>::abs_(((oy > zero) ? (dys[iy] - y) : y) / oy);                // :544-548
+        dby = Tm<T>::abs_(((oy > zero) ? (cdy - y) : y) / oy);                    // :544-548
         if (Tm<T>::nan_(dbx)) db = dby;                                       // :551-557
         else if (Tm<T>::nan_(dby)) db = dbx;
         else db = jmin(dbx, dby);
@@ -1164,7 +1183,7 @@ __global__ __launch_bounds__(TPB) void k_transport(SoA s, Geo g, Scalars* sc, vo
       const T d = jmin(jmin(db, dc), dcen);                                   // :92 / :571
       const W tau = csd * (W)d;
       const W Enew = E * Tm<W>::exp_(-tau);                                   // :95 / :580
-      const W Evol = TWO_D ? (E * cidx) * idyw[iy] : E * cidx;
+      const W Evol = TWO_D ? (E * cidx) * cidy : E * cidx;
       bin = k * C + cell;
       const bool capped = ev >= EV_MAX;
       ntrunc += capped ? 1u : 0u;
@@ -1257,9 +1276,20 @@ __global__ __launch_bounds__(TPB) void k_transport(SoA s, Geo g, Scalars* sc, vo
       ++ev;
     }
     if (SURR && s_ST < 0.f) has = true;     // a parked lane keeps its particle
-    tal.add(bin, val);
+    int fbin = -1;
+    Acc fval = (Acc)0;
+    if (bin >= 0) {
+      if (bin == abin) {
+        aval += (double)val;
+      } else {
+        fbin = abin; fval = (Acc)aval;
+        abin = bin; aval = (double)val;
+      }
+    }
+    tal.add(fbin, fval);
     tal.add(lbin, lval);
   }
+  tal.add(abin, (Acc)aval);
 
   segs = wave_sum(segs);
   cens = wave_sum(cens);
