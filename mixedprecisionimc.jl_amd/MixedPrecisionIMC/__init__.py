@@ -175,9 +175,11 @@ def step_device(inputs: dict, mesh, simvars: SimVars, particles: Engine, timeste
     mesh.lostenergy += r["lostenergy"]
     simvars.iterations += r["segments"]
     mesh.energyincrease_saved.append(particles.step_energy_increase().reshape(mesh.shape, order="F"))
-    mesh.temp_saved.append(mesh.temp.copy())
-    mesh.matenergy_saved.append(mesh.matenergydens.copy())
-    mesh.radenergy_saved.append(mesh.radenergydens.copy())
+    # the three fields are new arrays every step and are rebound, never written in place, so
+    # the saved histories keep them without a copy
+    mesh.temp_saved.append(mesh.temp)
+    mesh.matenergy_saved.append(mesh.matenergydens)
+    mesh.radenergy_saved.append(mesh.radenergydens)
     err = EnergyCheck.energychecker(inputs, mesh, simvars, particles, radenergy=r["radenergy"])
     Output.plotting(inputs, mesh, simvars)
     timestep(timestepping, simvars)

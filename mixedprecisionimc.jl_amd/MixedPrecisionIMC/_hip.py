@@ -338,7 +338,7 @@ class Engine:
 
     def step(self, step: int, dt: float, first: bool, n_input: int, n_max: int, cellmin: float):
         """imc_step: one device-resident step.  Returns (temp, radenergydens, matenergydens, result)."""
-        temp, rad, mat = np.zeros(self.C), np.zeros(self.C), np.zeros(self.C)
+        temp, rad, mat = np.empty(self.C), np.empty(self.C), np.empty(self.C)    # filled by imc_step
         r = ImcStepResult()
         self._check(lib().imc_step(self._h, int(step), float(dt), int(bool(first)), int(n_input), int(n_max),
                                    float(cellmin), _dp(temp), _dp(rad), _dp(mat), ctypes.byref(r)))
@@ -346,7 +346,7 @@ class Engine:
         return temp, rad, mat, {name: getattr(r, name) for name, _ in ImcStepResult._fields_}
 
     def step_energy_increase(self) -> np.ndarray:
-        inc = np.zeros(self.C)
+        inc = np.empty(self.C)
         self._check(lib().imc_step_energy_increase(self._h, _dp(inc)))
         return inc
 

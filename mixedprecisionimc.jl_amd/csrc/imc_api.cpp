@@ -138,6 +138,21 @@ size_t partials_needed(const Ctx& c) {
   return need;
 }
 
+// Per-tile bucket counts of the birth order (hl entries, with a quarter's headroom), followed in
+// the same allocation by the tile sums of their scan.
+int ensure_bhist(Ctx& c, int64_t hl) {
+  if (hl <= c.bhist_len) return IMC_OK;
+  (void)hipFree(c.d_bhist);
+  c.d_bhist = c.d_bhist_tiles = nullptr;
+  c.bhist_len = 0;
+  const int64_t len = hl + hl / 4;
+  if (dalloc(&c.d_bhist, sizeof(int64_t) * (size_t)(len + scan_tiles(len) + 1)) != hipSuccess)
+    return fail(c, IMC_ENOMEM, "birth-order scratch allocation failed");
+  c.d_bhist_tiles = c.d_bhist + len;
+  c.bhist_len = len;
+  return IMC_OK;
+}
+
 Geo make_geo(const Ctx& c) {
   Geo g;
   g.nx = c.nx; g.ny = c.ny; g.C = c.C; g.K = c.K; g.NB = c.NB;
@@ -220,6 +235,7 @@ int imc_create(imc_ctx** out, const imc_config* cfg) {
   ok &= dalloc(&c.d_ent_k, 4 * NE) == hipSuccess;
   ok &= dalloc(&c.d_ent_cnt, 8 * NE) == hipSuccess;
   ok &= dalloc(&c.d_ent_off, 8 * NE) == hipSuccess;
+  ok &= dalloc(&c.d_ent_tiles, 8 * (size_t)(scan_tiles(c.NE) + 1)) == hipSuccess;
   c.red_len = 2 * ((int64_t)NE / 256 + 4);
   ok &= dalloc(&c.d_red, 8 * (size_t)c.red_len) == hipSuccess;
   ok &= dalloc(&c.d_emitted, 8 * C * c.K) == hipSuccess;
@@ -248,7 +264,7 @@ int imc_destroy(imc_ctx* h) {
   free_soa(c.buf[0]); free_soa(c.buf[1]);
   void* ptrs[] = {c.d_dx, c.d_dy, c.d_dxs, c.d_dys, c.d_dxw, c.d_dyw, c.d_fields, c.d_sigcol, c.d_sigdep, c.d_isig, c.d_idxw, c.d_idyw, c.d_ent_e, c.d_ent_esc, c.d_ent_term,
                   c.d_ent_k, c.d_ent_cnt, c.d_ent_off, c.d_red, c.d_emitted, c.d_dep, c.d_gtally, c.d_rad, c.d_sc,
-                  c.d_partial, c.d_tile, c.d_bhist, c.d_surr_feat, c.d_surr_targ, c.d_surr_params, c.d_surr_grad,
+                  c.d_partial, c.d_tile, c.d_bhist, c.d_ent_tiles, c.d_surr_feat, c.d_surr_targ, c.d_surr_params, c.d_surr_grad,
                   c.d_surr_m, c.d_surr_v, c.d_surr_wbf, c.d_surr_bias, c.d_surr_loss, c.d_surr_events, c.d_surr_cells,
                   c.d_mat, c.d_rw_table};
   for (void* p : ptrs) (void)hipFree(p);
@@ -368,15 +384,8 @@ int imc_source(imc_ctx* h, int64_t n_input, int64_t n_max, double cellmin, doubl
       return fail(c, IMC_ENUMERIC, "total source energy is zero or not finite");
     rc = grow(c, c.h_sc.n_census + c.h_sc.n_births, c.h_sc.n_census);
     if (rc) return rc;
-    const int64_t hl = birth_hist_len(c.h_sc.n_births);
-    if (hl > c.bhist_len) {                          // per-tile bucket counts of the birth order
-      (void)hipFree(c.d_bhist);
-      c.d_bhist = nullptr;
-      c.bhist_len = 0;
-      if (dalloc(&c.d_bhist, sizeof(int64_t) * (size_t)(hl + hl / 4)) != hipSuccess)
-        return fail(c, IMC_ENOMEM, "birth-order scratch allocation failed");
-      c.bhist_len = hl + hl / 4;
-    }
+    rc = ensure_bhist(c, birth_hist_len(c.h_sc.n_births));
+    if (rc) return rc;
     HIPCHK(launch_births(c, c.h_sc.n_births), "birth kernel");
   }
   if (emittedenergy)
@@ -632,7 +641,7 @@ int imc_set_material(imc_ctx* h, const imc_material* m) {
     ts = {m->t_surf[0][0], m->t_surf[1][0]};
   }
   if (!c.d_mat) {
-    if (dalloc(&c.d_mat, 8 * MAT_N * C + sizeof(Scalars)) != hipSuccess)
+    if (dalloc(&c.d_mat, 8 * MAT_N * C + sizeof(Scalars) + step_sums_scratch_bytes(c)) != hipSuccess)
       return fail(c, IMC_ENOMEM, "material allocation failed");
     if (hipHostMalloc((void**)&c.h_out, 8 * MAT_OUT * C + sizeof(Scalars), hipHostMallocDefault) != hipSuccess) {
       c.h_out = nullptr;
@@ -711,15 +720,8 @@ int imc_step(imc_ctx* h, int64_t step, double dt, int32_t first, int64_t n_input
       return fail(c, IMC_ENUMERIC, "total source energy is zero or not finite");
     rc = grow(c, c.h_sc.n_census + c.h_sc.n_births, c.h_sc.n_census);
     if (rc) return rc;
-    const int64_t hl = birth_hist_len(c.h_sc.n_births);
-    if (hl > c.bhist_len) {
-      (void)hipFree(c.d_bhist);
-      c.d_bhist = nullptr;
-      c.bhist_len = 0;
-      if (dalloc(&c.d_bhist, sizeof(int64_t) * (size_t)(hl + hl / 4)) != hipSuccess)
-        return fail(c, IMC_ENOMEM, "birth-order scratch allocation failed");
-      c.bhist_len = hl + hl / 4;
-    }
+    rc = ensure_bhist(c, birth_hist_len(c.h_sc.n_births));
+    if (rc) return rc;
     HIPCHK(launch_births(c, c.h_sc.n_births), "birth kernel");
     n_bound = c.h_sc.n_census + c.h_sc.n_births;
   }

@@ -134,6 +134,8 @@ struct Ctx {
   double* d_rad = nullptr;         // C
   int64_t* d_tile = nullptr;       // clean: per-tile counts/offsets
   int64_t* d_bhist = nullptr;      // births: per-tile start-time bucket counts/offsets
+  int64_t* d_bhist_tiles = nullptr;  // scan tile sums of d_bhist (inside the same allocation)
+  int64_t* d_ent_tiles = nullptr;  // scan tile sums of the birth counts (scan_tiles(NE) + 1)
   int64_t bhist_len = 0;
   int64_t tile_len = 0;
 
@@ -204,8 +206,11 @@ int clean_choose(const Ctx& c, int* grid_blocks, int* tally_mode, size_t* lds_by
 // Device-resident step (imc_step.hip).
 hipError_t launch_update(Ctx& c, int first);
 hipError_t launch_temperature(Ctx& c);
+int64_t scan_tiles(int64_t n);
+hipError_t launch_scan_i64(Ctx& c, const int64_t* a, int64_t n, int64_t* out, int64_t* tiles);
 hipError_t launch_planck(Ctx& c, int64_t n, const double* d_temp, double* d_freq, uint32_t stream);
 hipError_t launch_step_sums(Ctx& c);
+size_t step_sums_scratch_bytes(const Ctx& c);   // after the scalar snapshot in d_mat
 hipError_t surr_generate(Ctx& c, int64_t n, uint32_t seed, unsigned long long* d_events);
 hipError_t surr_make_weights(Ctx& c);
 hipError_t surr_grad(Ctx& c, const float* feat, const float* targ, int64_t n0, int64_t n, float inv_batch,

@@ -4,7 +4,7 @@
 //   k_source_entries  emission / radiation-source / surface energies via the sorter
 //   k_tree256         deterministic FP64 tree sum (mesh.totalenergy)
 //   k_counts          particle budget and per-entry birth counts
-//   k_scan_counts     exclusive scan of birth counts (single block)
+//   k_scan_tile_sums, k_scan_tiles, k_counts_total: multi-block exclusive scan of birth counts
 //   k_births          one thread per birth: position, direction, time, weight
 //   k_transport       persistent event loop, one lane per history with lane refill,
 //                     wave-reduced tallies
@@ -287,33 +287,96 @@ __global__ void k_counts(int nx, int ny, int C, int64_t NE, int quirks, int64_t 
 }
 
 // Single-block exclusive scan of the birth counts; sets n_births / n_total.
-__global__ __launch_bounds__(1024) void k_scan_counts(const int64_t* __restrict__ cnt, int64_t n,
-                                                      int64_t* __restrict__ off, Scalars* sc, int64_t cap) {
-  __shared__ int64_t wtot[16];
-  __shared__ int64_t running;
-  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
-  if (tid == 0) running = 0;
-  __syncthreads();
-  for (int64_t base = 0; base < n; base += 1024) {
-    const int64_t i = base + tid;
-    const int64_t v = i < n ? cnt[i] : 0;
-    int64_t incl = v;
+// ---------------------------------------------------------------------------------
+// Exclusive scan of int64 counts over many blocks: tile sums (k_scan_tile_sums), a
+// single-block scan of the tile sums (k_exscan_i64, with the grand total left in entry
+// ntiles), then each tile scanned from its offset (k_scan_tiles; in place allowed).
+// ---------------------------------------------------------------------------------
+constexpr int SCAN_TPB = 256, SCAN_PER = 16, SCAN_TILE = SCAN_TPB * SCAN_PER;
+
+int64_t scan_tiles(int64_t n) { return (n + SCAN_TILE - 1) / SCAN_TILE; }
+
+__device__ __forceinline__ int64_t block_sum_i64(int64_t v, int64_t* wtot) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
 #pragma unroll
-    for (int d = 1; d < 64; d <<= 1) {
-      const int64_t o = __shfl_up(incl, d, 64);
-      if (lane >= d) incl += o;
-    }
-    if (lane == 63) wtot[w] = incl;
-    __syncthreads();
-    int64_t wpre = 0;
-    for (int j = 0; j < w; ++j) wpre += wtot[j];
-    const int64_t r0 = running;
-    if (i < n) off[i] = r0 + wpre + incl - v;
-    __syncthreads();
-    if (tid == 1023) running = r0 + wpre + incl;
-    __syncthreads();
+  for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d, 64);
+  if (lane == 0) wtot[w] = v;
+  __syncthreads();
+  int64_t s = 0;
+  for (int j = 0; j < SCAN_TPB / 64; ++j) s += wtot[j];
+  return s;
+}
+
+__global__ __launch_bounds__(SCAN_TPB) void k_scan_tile_sums(const int64_t* __restrict__ a, int64_t n,
+                                                              int64_t* __restrict__ tsum) {
+  __shared__ int64_t wtot[SCAN_TPB / 64];
+  const int64_t base = (int64_t)blockIdx.x * SCAN_TILE;
+  int64_t v = 0;
+#pragma unroll
+  for (int j = 0; j < SCAN_PER; ++j) {
+    const int64_t i = base + j * SCAN_TPB + threadIdx.x;
+    if (i < n) v += a[i];
   }
-  if (tid == 0) {
+  const int64_t s = block_sum_i64(v, wtot);
+  if (threadIdx.x == 0) tsum[blockIdx.x] = s;
+}
+
+__global__ __launch_bounds__(SCAN_TPB) void k_scan_tiles(const int64_t* a, int64_t n, const int64_t* __restrict__ toff,
+                                                          int64_t* out) {
+  __shared__ int64_t tile[SCAN_TILE];
+  __shared__ int64_t wtot[SCAN_TPB / 64];
+  const int64_t base = (int64_t)blockIdx.x * SCAN_TILE;
+#pragma unroll
+  for (int j = 0; j < SCAN_PER; ++j) {                 // coalesced load of the tile
+    const int64_t i = base + j * SCAN_TPB + threadIdx.x;
+    tile[j * SCAN_TPB + threadIdx.x] = i < n ? a[i] : 0;
+  }
+  __syncthreads();
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  int64_t* mine = tile + threadIdx.x * SCAN_PER;       // each thread owns SCAN_PER consecutive entries
+  int64_t sum = 0;
+#pragma unroll
+  for (int j = 0; j < SCAN_PER; ++j) sum += mine[j];
+  int64_t incl = sum;
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const int64_t o = __shfl_up(incl, d, 64);
+    if (lane >= d) incl += o;
+  }
+  if (lane == 63) wtot[w] = incl;
+  __syncthreads();
+  int64_t run = toff[blockIdx.x] + incl - sum;
+  for (int j = 0; j < w; ++j) run += wtot[j];
+#pragma unroll
+  for (int j = 0; j < SCAN_PER; ++j) {
+    const int64_t v = mine[j];
+    mine[j] = run;
+    run += v;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int j = 0; j < SCAN_PER; ++j) {
+    const int64_t i = base + j * SCAN_TPB + threadIdx.x;
+    if (i < n) out[i] = tile[j * SCAN_TPB + threadIdx.x];
+  }
+}
+
+__global__ __launch_bounds__(1024) void k_exscan_i64(int64_t* __restrict__ a, int64_t n);
+
+// out[i] = a[0] + ... + a[i-1] for i < n, and tiles[scan_tiles(n)] = the total; tiles holds
+// scan_tiles(n) + 1 entries.  n >= 1.
+hipError_t launch_scan_i64(Ctx& c, const int64_t* a, int64_t n, int64_t* out, int64_t* tiles) {
+  const int64_t nt = scan_tiles(n);
+  k_scan_tile_sums<<<(unsigned)nt, SCAN_TPB, 0, c.stream>>>(a, n, tiles);
+  k_exscan_i64<<<1, 1024, 0, c.stream>>>(tiles, nt + 1);     // entry nt (any value) ends as the total
+  k_scan_tiles<<<(unsigned)nt, SCAN_TPB, 0, c.stream>>>(a, n, tiles, out);
+  return hipGetLastError();
+}
+
+// Birth total of the scanned counts (imc_sourcing.jl:126-158 budget), with the slot check.
+__global__ void k_counts_total(const int64_t* __restrict__ total, Scalars* sc, int64_t cap) {
+  if (threadIdx.x == 0) {
+    const int64_t running = *total;
     if (running < 0 || running > cap - sc->n_census) {   // more births than slots: none, flagged
       sc->n_births = 0;
       sc->n_total = sc->n_census;
@@ -338,7 +401,9 @@ hipError_t launch_counts_and_scan(Ctx& c, int64_t n_input, int64_t n_max, double
 #undef CNT
   hipError_t err = hipGetLastError();
   if (err != hipSuccess) return err;
-  k_scan_counts<<<1, 1024, 0, c.stream>>>(c.d_ent_cnt, c.NE, c.d_ent_off, c.d_sc, cap);
+  err = launch_scan_i64(c, c.d_ent_cnt, c.NE, c.d_ent_off, c.d_ent_tiles);
+  if (err != hipSuccess) return err;
+  k_counts_total<<<1, 64, 0, c.stream>>>(c.d_ent_tiles + scan_tiles(c.NE), c.d_sc, cap);
   return hipGetLastError();
 }
 
@@ -585,20 +650,23 @@ hipError_t launch_births(Ctx& c, int64_t n_births) {
   A.ent_nrg = c.d_ent_e; A.ent_esc = c.d_ent_esc; A.ent_k = c.d_ent_k; A.ent_off = c.d_ent_off;
   A.sc = c.d_sc; A.out = c.buf[c.cur];
   const int64_t ntiles = (n_births + BIRTH_TILE - 1) / BIRTH_TILE;
-  if ((int64_t)BK * ntiles > c.bhist_len) return hipErrorInvalidValue;   // sized by the caller
+  if ((int64_t)BK * ntiles > c.bhist_len) return hipErrorInvalidValue;   // sized by the caller (ensure_bhist)
   const unsigned grid = (unsigned)ntiles;
   const size_t elds = c.NE <= ENT_LDS_MAX ? sizeof(int64_t) * (size_t)c.NE : 0;
+  hipError_t err = hipSuccess;
 #define BIR(P, D)                                                                               \
   do {                                                                                          \
     k_birth_hist<P, D><<<grid, 256, elds, c.stream>>>(A, ntiles, c.d_bhist);                    \
-    k_exscan_i64<<<1, 1024, 0, c.stream>>>(c.d_bhist, (int64_t)BK * ntiles);                    \
+    err = launch_scan_i64(c, c.d_bhist, (int64_t)BK * ntiles, c.d_bhist, c.d_bhist_tiles);      \
+    if (err != hipSuccess) break;                                                               \
     k_births<P, D><<<grid, 256, elds, c.stream>>>(A, ntiles, c.d_bhist);                        \
+    err = hipGetLastError();                                                                    \
   } while (0)
   if (c.prec == 64) { if (c.is2d) BIR(64, true); else BIR(64, false); }
   else if (c.prec == 32) { if (c.is2d) BIR(32, true); else BIR(32, false); }
   else { if (c.is2d) BIR(16, true); else BIR(16, false); }
 #undef BIR
-  return hipGetLastError();
+  return err;
 }
 
 int64_t birth_hist_len(int64_t n_births) {

@@ -8,6 +8,8 @@
 // no transcendental is involved, the same inputs give the host's values bit for bit.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "imc_device.h"
 #include "imc_internal.h"
 
@@ -125,30 +127,56 @@ hipError_t launch_temperature(Ctx& c) {
 // One block: E_dep = sum_k sum_i energydep_k[i] * vol_i / scale_k (summed per scale, then over
 // scales in order, as imc_tally.jl:47-57), E_rad = sum_i radenergydens[i] * vol_i, and the lost
 // bin.  Strided per-thread partials and a fixed tree: the result does not depend on timing.
-__global__ __launch_bounds__(1024) void k_step_sums(int C, int K, int nx, Geo g, const double* __restrict__ dx,
-                                                    const double* __restrict__ dy, const double* __restrict__ dep,
-                                                    const double* __restrict__ rad, Scalars* sc, Scalars* snap) {
-  __shared__ double part[1024];
+// Two passes: SUMS_BLOCKS blocks each reduce a grid-strided share of the cells for every scale
+// and for the radiation energy; one block then adds the block partials in a fixed order (the
+// sums are deterministic) and writes the scalars.
+constexpr int SUMS_BLOCKS = 256, SUMS_TPB = 256;
+
+size_t step_sums_scratch_bytes(const Ctx& c) { return 8 * (size_t)(c.K + 1) * SUMS_BLOCKS; }
+
+__device__ __forceinline__ double block_tree_sum(double v, double* red) {
   const int tid = threadIdx.x;
-  double edep = 0.0;
-  for (int k = 0; k <= K; ++k) {
-    double acc = 0.0;
-    for (int i = tid; i < C; i += 1024) {      // host: dep * (dx dy) / scale and rad * dx * dy
-      const double dxi = dx[i % nx], dyi = dy[i / nx];
-      acc += k < K ? dep[(size_t)k * C + i] * (dxi * dyi) / g.scales[k] : rad[i] * dxi * dyi;
-    }
-    part[tid] = acc;
-    __syncthreads();
-    for (int h = 512; h > 0; h >>= 1) {
-      if (tid < h) part[tid] += part[tid + h];
-      __syncthreads();
-    }
-    if (k < K) edep += part[0];
-    else if (tid == 0) sc->E_rad = part[0];
+  red[tid] = v;
+  __syncthreads();
+  for (int h = SUMS_TPB / 2; h > 0; h >>= 1) {
+    if (tid < h) red[tid] += red[tid + h];
     __syncthreads();
   }
-  if (tid == 0) {
+  const double s = red[0];
+  __syncthreads();
+  return s;
+}
+
+__global__ __launch_bounds__(SUMS_TPB) void k_step_sums_part(int C, int K, int nx, Geo g, const double* __restrict__ dx,
+                                                             const double* __restrict__ dy,
+                                                             const double* __restrict__ dep,
+                                                             const double* __restrict__ rad, double* __restrict__ part) {
+  __shared__ double red[SUMS_TPB];
+  const int stride = gridDim.x * SUMS_TPB;
+  for (int k = 0; k <= K; ++k) {
+    double acc = 0.0;
+    for (int i = blockIdx.x * SUMS_TPB + threadIdx.x; i < C; i += stride) {
+      const double dxi = dx[i % nx], dyi = dy[i / nx];       // host: dep * (dx dy) / scale, rad * dx * dy
+      acc += k < K ? dep[(size_t)k * C + i] * (dxi * dyi) / g.scales[k] : rad[i] * dxi * dyi;
+    }
+    const double s = block_tree_sum(acc, red);
+    if (threadIdx.x == 0) part[(size_t)k * gridDim.x + blockIdx.x] = s;
+  }
+}
+
+__global__ __launch_bounds__(SUMS_TPB) void k_step_sums(int C, int K, int nblocks, const double* __restrict__ part,
+                                                        const double* __restrict__ dep, Scalars* sc, Scalars* snap) {
+  __shared__ double red[SUMS_TPB];
+  double edep = 0.0, erad = 0.0;
+  for (int k = 0; k <= K; ++k) {
+    double acc = 0.0;
+    for (int b = threadIdx.x; b < nblocks; b += SUMS_TPB) acc += part[(size_t)k * nblocks + b];
+    const double s = block_tree_sum(acc, red);
+    if (k < K) edep += s; else erad = s;
+  }
+  if (threadIdx.x == 0) {
     sc->E_dep = edep;
+    sc->E_rad = erad;
     sc->E_lost = dep[(size_t)K * C];
     *snap = *sc;               // the step's scalars, next to its arrays for one copy back
     // and the counters the next step needs at zero (no fill launches there)
@@ -159,8 +187,11 @@ __global__ __launch_bounds__(1024) void k_step_sums(int C, int K, int nx, Geo g,
 
 hipError_t launch_step_sums(Ctx& c) {
   const Geo g = make_geo(c);
-  k_step_sums<<<1, 1024, 0, c.stream>>>(c.C, c.K, c.nx, g, c.d_dx, c.d_dy, c.d_dep, c.d_rad, c.d_sc,
-                                        (Scalars*)(c.d_mat + (size_t)MAT_N * c.C));
+  Scalars* snap = (Scalars*)(c.d_mat + (size_t)MAT_N * c.C);
+  double* part = (double*)(snap + 1);
+  const int nblocks = (int)std::min<int64_t>(SUMS_BLOCKS, (c.C + SUMS_TPB - 1) / SUMS_TPB);
+  k_step_sums_part<<<nblocks, SUMS_TPB, 0, c.stream>>>(c.C, c.K, c.nx, g, c.d_dx, c.d_dy, c.d_dep, c.d_rad, part);
+  k_step_sums<<<1, SUMS_TPB, 0, c.stream>>>(c.C, c.K, nblocks, part, c.d_dep, c.d_sc, snap);
   return hipGetLastError();
 }
 

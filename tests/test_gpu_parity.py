@@ -55,6 +55,27 @@ def test_source_matches_oracle(prec, geom):
     e.close()
 
 
+@pytest.mark.parametrize("geom", [1, 2])
+def test_many_births_match_oracle(geom):
+    """Enough births (6e5: 586 birth tiles x 16 start-time buckets = 9,376 histogram entries)
+    that the birth-order scan spans several scan tiles of 4,096; the 2D mesh of 100 x 90 cells
+    has 18,380 count entries, so the count scan does too.  Births equal the oracle's bit for
+    bit."""
+    case = case1d(prec=32) if geom == 1 else case2d(nx=100, ny=90, prec=32)
+    print(f"[many births] geometry={geom}", flush=True)
+    e = engine(case, capacity=1_500_000)
+    engine_step(e, case, 3)
+    emitted, etot, n = e.source(600000, 6000000, 2.0)
+    births = e.export_particles()
+    op, st = oracle_problem(case)
+    rec, o_emit, o_etot, nb = op.source(st, 3, None, 600000, 6000000, 2.0)
+    assert n == nb and n > 4096 * 1024 // 16, (n, nb)
+    np.testing.assert_array_equal(_cells(births, geom), _cells(rec, geom))
+    for name, idx in _fields(geom):
+        np.testing.assert_array_equal(births[:, idx], rec[:, idx], err_msg=name)
+    e.close()
+
+
 def _replay(case, n_input=3000, step=2, **tuning):
     e = engine(case, **tuning)
     op, st = oracle_problem(case)
