@@ -8,6 +8,7 @@
 #include <cstddef>
 #include <cstdio>
 #include <cstring>
+#include <functional>
 #include <new>
 #include <random>
 #include <vector>
@@ -1023,8 +1024,6 @@ static double rw_exit_cdf(double a) {
   return std::min(std::max(f, 0.0), 1.0);
 }
 
-// Transport.MC_RW (imc_transport.jl:212-479) with the random-walk table of :786-797, as a
-// correct analytic walk (see the RW note in imc_kernels.hip).
 int imc_sample_planck(imc_ctx* h, int64_t n, const double* temp, uint32_t stream, double* freq) {
   if (!h) return IMC_EINVAL;
   Ctx& c = h->c;
@@ -1048,28 +1047,68 @@ int imc_sample_planck(imc_ctx* h, int64_t n, const double* temp, uint32_t stream
   return IMC_OK;
 }
 
+// First exit of two-dimensional diffusion from the centre of the unit disk, in a = D t / R0^2:
+// F(a) = 1 - sum_k c_k e^{-j_k^2 a}, c_k = 2 / (j_k J1(j_k)), j_k the zeros of J0 (eigenfunction
+// expansion of the disk with an absorbing rim).  DISK_TERMS terms converge for a >= 3.2e-5, below
+// which F < 1e-300 is taken as 0.
+constexpr int DISK_TERMS = 400;
+
+struct DiskSeries {
+  double j[DISK_TERMS], c[DISK_TERMS];
+  DiskSeries() {
+    const double pi = 3.14159265358979323846;
+    for (int k = 0; k < DISK_TERMS; ++k) {
+      const double beta = (k + 0.75) * pi;                   // McMahon's estimate of the (k+1)-th zero
+      double x = beta + 1.0 / (8.0 * beta);
+      for (int it = 0; it < 8; ++it) x += std::cyl_bessel_j(0.0, x) / std::cyl_bessel_j(1.0, x);   // J0' = -J1
+      j[k] = x;
+      c[k] = 2.0 / (x * std::cyl_bessel_j(1.0, x));
+    }
+  }
+  double exit_cdf(double a) const {
+    if (a < 3.2e-5) return 0.0;
+    double s = 0.0;
+    for (int k = 0; k < DISK_TERMS; ++k) s += c[k] * std::exp(-j[k] * j[k] * a);
+    return std::min(std::max(1.0 - s, 0.0), 1.0);
+  }
+};
+
+// Inverse of a CDF on [lo, hi] at u by bisection.
+static double invert_cdf(const std::function<double(double)>& cdf, double u, double lo, double hi) {
+  for (int it = 0; it < 100; ++it) {
+    const double mid = 0.5 * (lo + hi);
+    if (cdf(mid) < u) lo = mid; else hi = mid;
+  }
+  return 0.5 * (lo + hi);
+}
+
+// Transport.MC_RW (imc_transport.jl:212-479) with the random-walk table of :786-797, as a
+// correct analytic walk (see the RW notes in imc_kernels.hip).  Tables: the exit time's inverse
+// CDF at u = i / RW_TABLE_POINTS (the tail past (N-1)/N uses the closed form); in 2D followed by
+// the census radius's inverse CDF rho J1(j1 rho) / J1(j1) at u = i / (RW_RADIUS_POINTS - 1).
 int imc_random_walk_enable(imc_ctx* h, int32_t on, double r_min) {
   if (!h) return IMC_EINVAL;
   Ctx& c = h->c;
-  if (on && c.is2d) return fail(c, IMC_EINVAL, "the random walk is available for 1D geometry");
   if (on && !(r_min >= 1.0 && std::isfinite(r_min)))
     return fail(c, IMC_EINVAL, "r_min must be at least one mean free path");
   if (on && !c.d_rw_table) {
-    // inverse CDF at u = i / N by bisection; the tail past (N-1)/N uses the closed form
-    std::vector<float> tab(RW_TABLE_POINTS);
-    for (int i = 0; i < RW_TABLE_POINTS; ++i) {
-      const double u = (double)i / RW_TABLE_POINTS;
-      double lo = 0.0, hi = 60.0;
-      for (int it = 0; it < 100; ++it) {
-        const double mid = 0.5 * (lo + hi);
-        if (rw_exit_cdf(mid) < u) lo = mid; else hi = mid;
-      }
-      tab[i] = (float)(0.5 * (lo + hi));
+    const size_t n = RW_TABLE_POINTS + (c.is2d ? RW_RADIUS_POINTS : 0);
+    std::vector<float> tab(n);
+    if (!c.is2d) {
+      for (int i = 0; i < RW_TABLE_POINTS; ++i)
+        tab[i] = (float)invert_cdf(rw_exit_cdf, (double)i / RW_TABLE_POINTS, 0.0, 60.0);
+    } else {
+      const DiskSeries d;
+      for (int i = 0; i < RW_TABLE_POINTS; ++i)
+        tab[i] = (float)invert_cdf([&](double a) { return d.exit_cdf(a); }, (double)i / RW_TABLE_POINTS, 0.0, 40.0);
+      const double j1 = d.j[0], J1j1 = std::cyl_bessel_j(1.0, j1);
+      const auto rad_cdf = [&](double r) { return r * std::cyl_bessel_j(1.0, j1 * r) / J1j1; };
+      for (int i = 0; i < RW_RADIUS_POINTS; ++i)
+        tab[RW_TABLE_POINTS + i] = (float)invert_cdf(rad_cdf, (double)i / (RW_RADIUS_POINTS - 1), 0.0, 1.0);
     }
-    if (dalloc(&c.d_rw_table, sizeof(float) * RW_TABLE_POINTS) != hipSuccess)
+    if (dalloc(&c.d_rw_table, sizeof(float) * n) != hipSuccess)
       return fail(c, IMC_ENOMEM, "random-walk table allocation failed");
-    HIPCHK(hipMemcpy(c.d_rw_table, tab.data(), sizeof(float) * RW_TABLE_POINTS, hipMemcpyHostToDevice),
-           "random-walk table H2D");
+    HIPCHK(hipMemcpy(c.d_rw_table, tab.data(), sizeof(float) * n, hipMemcpyHostToDevice), "random-walk table H2D");
   }
   c.rw_enabled = on != 0;
   if (on) {

@@ -56,6 +56,7 @@ struct Scalars {
 };
 
 constexpr int RW_TABLE_POINTS = 4096;   // analytic random walk: inverse-CDF table of the exit time
+constexpr int RW_RADIUS_POINTS = 1024;  // 2D walk: inverse CDF of the census radius (fundamental mode)
 
 struct SurrBox {               // training box of the thick-cell surrogate (see imc_mlp.h)
   float tau_cell_min, tau_cell_max;

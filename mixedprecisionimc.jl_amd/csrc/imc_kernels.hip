@@ -884,6 +884,73 @@ __device__ __forceinline__ float rw_census_y(float a, float u, uint32_t slot, ui
   return y;
 }
 
+// ---------------------------------------------------------------------------------
+// RW (2D): the same walk in the plane.  Directions are angles in the plane (imc_sourcing.jl
+// :272-362, imc_transport.jl:710), so effective scattering diffuses with D = c / (2 sigma_c),
+// and the walk is taken over the disk of radius R0 = the distance to the nearest face: in path
+// units a = s / (2 sigma_c R0^2).  The first exit from the disk's centre has
+// P(T <= a) = 1 - sum_k 2 e^{-j_k^2 a} / (j_k J1(j_k)) (j_k the zeros of J0; an exact table of its
+// inverse CDF, with the one-term tail a = ln(c1 / (1 - u)) / j1^2), at a uniform point of the
+// circle, in a direction psi from the outward normal with density cos psi (1 + (4/pi) cos psi) / 4
+// (the P1 emergent law of the plane, as mu (1 + 1.5 mu) is in 1D).  A walker still inside at
+// census is placed by the disk's conditional density: radius from the fundamental mode
+// rho J0(j1 rho) (a table of its inverse CDF rho J1(j1 rho) / J1(j1)) for a >= 0.2, else a
+// Gaussian of variance 2a per axis with the first-order image of the nearest boundary, accepted
+// with probability 1 - e^{-(1 - rho) / a}; the angle is uniform.  The circle lies inside the cell,
+// so the walk never crosses a face.  Weight, deposit and cut-off as in 1D.
+// ---------------------------------------------------------------------------------
+__device__ __forceinline__ float rw2_exit_a(const float* __restrict__ tab, float u) {
+  const float f = u * (float)RW_TABLE_POINTS;
+  const int i = (int)f;
+  // 1 - u = c1 e^{-j1^2 a}, c1 = 2 / (j1 J1(j1)) = 1.6019747, 1 / j1^2 = 0.17291507
+  if (i >= RW_TABLE_POINTS - 1) return 0.17291507f * __logf(1.6019747f / fmaxf(1.0f - u, 0x1p-30f));
+  const float w = f - (float)i;
+  return tab[i] + w * (tab[i + 1] - tab[i]);
+}
+
+// Offset of a census walker from the disk's centre, in units of R0.
+__device__ __forceinline__ float2 rw2_census(float a, const float* __restrict__ rad, float u, float v, uint32_t slot,
+                                             uint32_t ev, uint32_t seed, uint32_t step) {
+  if (a >= 0.2f) {
+    const float f = u * (float)(RW_RADIUS_POINTS - 1);
+    const int i = min((int)f, RW_RADIUS_POINTS - 2);
+    const float rho = rad[i] + (f - (float)i) * (rad[i + 1] - rad[i]);
+    float sn, cs;
+    __sincosf(6.2831853f * v, &sn, &cs);
+    return make_float2(rho * cs, rho * sn);
+  }
+  const float sg = sqrtf(2.0f * a);
+  float2 p = make_float2(0.0f, 0.0f);
+  for (uint32_t at = 1; at <= 16; ++at) {
+    const u4 r = philox(slot, ev, TAG_RW, at, seed, step);
+    const float m = sg * sqrtf(-2.0f * __logf((float)((r.r[0] >> 8) + 1) * 0x1p-24f));
+    float sn, cs;
+    __sincosf(6.2831853f * u24(r.r[1]), &sn, &cs);
+    const float rho = m;                                  // |(m cos, m sin)|
+    if (rho >= 1.0f) continue;
+    p = make_float2(m * cs, m * sn);
+    if (u24(r.r[2]) < 1.0f - __expf(-(1.0f - rho) / a)) break;
+  }
+  return p;
+}
+
+// Exit direction from the outward normal: psi in (-pi/2, pi/2) with density
+// cos psi (1 + (4/pi) cos psi) / 4, CDF (1 + sin psi + (2/pi)(psi + pi/2) + sin(2 psi) / pi) / 4,
+// inverted by Newton's method from the cosine law's inverse asin(2u - 1).
+__device__ __forceinline__ float rw2_exit_psi(float u) {
+  float psi = asinf(2.0f * u - 1.0f);
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    float sn, cs;
+    __sincosf(psi, &sn, &cs);
+    const float F = 0.25f * (1.0f + sn + 0.63661977f * (psi + 1.5707963f) + 0.31830989f * (2.0f * sn * cs));
+    const float pdf = 0.25f * cs * (1.0f + 1.2732395f * cs);
+    psi -= (F - u) / fmaxf(pdf, 1e-6f);
+    psi = fminf(fmaxf(psi, -1.5707963f), 1.5707963f);
+  }
+  return psi;
+}
+
 template <int P, int TM, bool TWO_D, int ACC, bool CELL>
 __global__ __launch_bounds__(TPB) void k_transport(SoA s, Geo g, Scalars* sc, void* partial_v,
                                                    double* gbins, SurrArgs sa) {
@@ -891,7 +958,7 @@ __global__ __launch_bounds__(TPB) void k_transport(SoA s, Geo g, Scalars* sc, vo
   using W = typename Prec<P>::W;
   using Acc = typename Prec<P>::Acc;
   constexpr bool SURR = ACC == 1, RW = ACC == 2;       // learned surrogate / analytic random walk
-  static_assert(!((SURR || RW) && TWO_D), "the surrogate and the random walk are 1D");
+  static_assert(!(SURR && TWO_D), "the surrogate is 1D");
   static_assert(!(CELL && TWO_D), "the LDS cell table is 1D");
   extern __shared__ __align__(16) unsigned char smem[];
   Acc* lds = (Acc*)smem;
@@ -1048,12 +1115,67 @@ __global__ __launch_bounds__(TPB) void k_transport(SoA s, Geo g, Scalars* sc, vo
     T rwR0 = zero;
     if (RW && has) {
       const T cdx_ = dx_of(ix);
-      const T cis_ = CELL ? s_cell[ix].isig : isig[ix];
+      const T cis_ = CELL ? s_cell[ix].isig : isig[TWO_D ? ix + nx * iy : ix];
       rwR0 = (x < cdx_ - x) ? x : cdx_ - x;
+      if (TWO_D) {
+        const T cdy_ = dys[iy];
+        const T ry = (y < cdy_ - y) ? y : cdy_ - y;
+        rwR0 = (ry < rwR0) ? ry : rwR0;
+      }
       rw = (float)rwR0 > 0.0f && (float)rwR0 >= sa.rw_min * (float)cis_;
     }
 
-    if (RW && rw) {
+    if (RW && TWO_D && rw) {
+      ++segs;
+      ++nsurr;
+      const u4 q = philox(slot, ev, TAG_RW, 0u, g.seed, g.step);
+      const int cell = ix + nx * iy;
+      bin = k * C + cell;
+      const float isf = (float)isig[cell], sdf = (float)sigdep[cell];
+      const W cidxy = idxw[ix] * idyw[iy];
+      const float R0 = (float)rwR0;
+      const float spa = 2.0f * R0 * R0 / isf;                  // path length per unit of a
+      const float s_e = rw2_exit_a(sa.rw_table, u24(q.r[0])) * spa;
+      const float s_rem = (float)(cT * (dtT - t) * sT);
+      const float s_cut = sdf > 0.f ? __logf((float)E / (float)Emin) / sdf : INFINITY;
+      if (s_cut < fminf(s_e, s_rem)) {                        // 1% cut-off first: all deposited
+        val = (Acc)(E * cidxy);
+        CK[slot] = DEAD;
+        has = false;
+      } else if (s_e < s_rem) {                                // leaves the disk, inside the cell
+        const W Enew = E * (W)__expf(-sdf * s_e);
+        val = (Acc)((E - Enew) * cidxy);
+        E = Enew;
+        t = t + Tm<T>::from((double)s_e) * icsT;
+        const float phi = 6.2831853f * u24(q.r[1]);
+        float sph, cph;
+        __sincosf(phi, &sph, &cph);
+        const T cdx_ = dxs[ix], cdy_ = dys[iy];
+        const T xn = x + Tm<T>::from((double)(R0 * cph)), yn = y + Tm<T>::from((double)(R0 * sph));
+        x = xn < zero ? zero : (xn > cdx_ ? cdx_ : xn);
+        y = yn < zero ? zero : (yn > cdy_ ? cdy_ : yn);
+        mu = Tm<T>::from((double)(phi + rw2_exit_psi(u24(q.r[2]))));
+        ox = Tm<T>::cos_(mu); oy = Tm<T>::sin_(mu);
+      } else {                                                 // census inside the disk
+        const W Enew = E * (W)__expf(-sdf * s_rem);
+        val = (Acc)((E - Enew) * cidxy);
+        E = Enew;
+        const float2 o = rw2_census(s_rem / spa, sa.rw_table + RW_TABLE_POINTS, u24(q.r[2]), u24(q.r[3]),
+                                    slot, ev, g.seed, g.step);
+        const float xn = fminf(fmaxf((float)x + R0 * o.x, 0.0f), (float)dxs[ix]);
+        const float yn = fminf(fmaxf((float)y + R0 * o.y, 0.0f), (float)dys[iy]);
+        const u4 q2 = philox(slot, ev, TAG_RW, 17u, g.seed, g.step);
+        CK[slot] = (uint32_t)cell | ((uint32_t)k << KSHIFT);
+        X[slot] = Tm<T>::from((double)xn);
+        Y[slot] = Tm<T>::from((double)yn);
+        MU[slot] = Tm<T>::from(TWO_PI_D * (double)u24(q2.r[0]));
+        TT[slot] = zero;
+        EE[slot] = E;
+        ++cens;
+        has = false;
+      }
+      ++ev;
+    } else if (RW && !TWO_D && rw) {
       ++segs;
       ++nsurr;
       const u4 q = philox(slot, ev, TAG_RW, 0u, g.seed, g.step);
@@ -1404,7 +1526,7 @@ static void pick_tally(const Ctx& c, int nbins, int* tally_mode, size_t* lds_byt
 }
 
 static bool use_surrogate(const Ctx& c) { return c.surr_enabled && !c.is2d; }
-static bool use_random_walk(const Ctx& c) { return c.rw_enabled && !c.is2d && !use_surrogate(c); }
+static bool use_random_walk(const Ctx& c) { return c.rw_enabled && !use_surrogate(c); }
 
 // Bytes of the 1D LDS cell table, 0 when it is not used (2D, or more than CELL_LDS_MAX).
 static size_t cell_table_bytes(const Ctx& c) {
@@ -1430,6 +1552,12 @@ static size_t cell_table_bytes(const Ctx& c) {
     if (tm == TM_BLOCKROW) F(P, TM_BLOCKROW, false, A, L);                               \
     else F(P, TM_GLOBAL, false, A, L);                                                   \
   } while (0)
+#define IMC_TM_DISPATCH_RW2(F, P)                                                        \
+  do {                                                                                   \
+    if (tm == TM_BLOCKROW) F(P, TM_BLOCKROW, true, 2, false);                            \
+    else if (tm == TM_HASH) F(P, TM_HASH, true, 2, false);                               \
+    else F(P, TM_GLOBAL, true, 2, false);                                                \
+  } while (0)
 #define IMC_TRANSPORT_DISPATCH_L(F, P, L)                                                \
   do {                                                                                   \
     if (surr_) IMC_TM_DISPATCH_ACC(F, P, 1, L);                                          \
@@ -1438,7 +1566,8 @@ static size_t cell_table_bytes(const Ctx& c) {
   } while (0)
 #define IMC_TRANSPORT_DISPATCH_P(F, P)                                                   \
   do {                                                                                   \
-    if (c.is2d) IMC_TM_DISPATCH(F, P, true, 0, false);                                   \
+    if (c.is2d && rw_) IMC_TM_DISPATCH_RW2(F, P);                                        \
+    else if (c.is2d) IMC_TM_DISPATCH(F, P, true, 0, false);                              \
     else if (cell_) IMC_TRANSPORT_DISPATCH_L(F, P, true);                                \
     else IMC_TRANSPORT_DISPATCH_L(F, P, false);                                          \
   } while (0)
@@ -1454,9 +1583,13 @@ static size_t cell_table_bytes(const Ctx& c) {
 
 int transport_choose(const Ctx& c, int* grid_blocks, int* tally_mode, size_t* lds_bytes) {
   pick_tally(c, c.NB, tally_mode, lds_bytes);
-  if ((use_surrogate(c) || use_random_walk(c)) && (*tally_mode == TM_WAVEROW || *tally_mode == TM_HASH)) {
+  const bool acc1d = use_surrogate(c) || (use_random_walk(c) && !c.is2d);
+  if (acc1d && (*tally_mode == TM_WAVEROW || *tally_mode == TM_HASH)) {
     *tally_mode = TM_GLOBAL;
     *lds_bytes = 0;
+  } else if (use_random_walk(c) && c.is2d && *tally_mode == TM_WAVEROW) {   // 2D walk: no wave rows
+    *tally_mode = TM_BLOCKROW;
+    *lds_bytes = ((size_t)c.NB * (c.prec == 64 ? 8 : 4) + 15) & ~(size_t)15;
   }
   *lds_bytes += cell_table_bytes(c);   // LDS layout: [tally][cell table][surrogate]
   const size_t total = *lds_bytes + (use_surrogate(c) ? SURR_LDS : 0);
