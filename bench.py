@@ -264,6 +264,69 @@ def large_2d_leg(warmup: int = 4, steps: int = 8):
                    "tally_mode": TALLY_MODES.get(tm["transport_tally_mode"]),
                    "max_abs_energy_error": max(abs(e) for e in simvars.energy_errors)}
     particles.close()
+    # the same window with the analytic random walk (at least 5 mean free paths from every face)
+    mesh, simvars, particles = M.setup(inp, precision=32)
+    particles.random_walk_enable(True, r_min=5.0)
+    for _ in range(warmup):
+        M.step_device(inp, mesh, simvars, particles, ts)
+    particles.synchronize()
+    particles.reset_timing()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        M.step_device(inp, mesh, simvars, particles, ts)
+    particles.synchronize()
+    wall = time.perf_counter() - t0
+    tm = particles.timing()
+    out["fp32_random_walk"] = {"histories_per_s": tm["histories"] / wall,
+                               "hot_path_histories_per_s": tm["histories"] / _hot_s(tm),
+                               "ms_per_step": 1e3 * wall / steps, "segments": tm["segments"],
+                               "walk_events": tm["surrogate_events"],
+                               "ms_transport_kernel": tm["ms_transport_kernel"],
+                               "max_abs_energy_error": max(abs(e) for e in simvars.energy_errors)}
+    particles.close()
+    out["fom"] = large_2d_fom(inp)
+    return out
+
+
+def large_2d_fom(inp, steps: int = 8, seeds: int = 3):
+    """Random walk against plain transport on the large 2D workload at equal error: after `steps`
+    steps, eps = relative RMS of the material temperature over the heated cells (rise over T_INIT
+    above 5% of the largest) against an FP64 plain reference (2x the particles, `seeds` seeds);
+    FOM = 1 / (eps^2 t), t = the run's GPU hot-path time (sourcing, transport, clean, census)."""
+    import numpy as np
+    import MixedPrecisionIMC as M
+
+    ts = str(inp["TIMESTEPPING"]).upper()
+    n_input = int(float(inp["NINPUT"]))
+
+    def run(prec, seed, walk, scale=1):
+        d = dict(inp)
+        d["NINPUT"] = str(scale * n_input)
+        d["NMAX"] = str(10 * scale * n_input)
+        mesh, simvars, particles = M.setup(d, precision=prec, seed=seed, capacity=3 * scale * n_input)
+        if walk:
+            particles.random_walk_enable(True, r_min=5.0)
+        for _ in range(steps):
+            M.step_device(d, mesh, simvars, particles, ts)
+        tm = particles.timing()
+        particles.close()
+        return np.asarray(mesh.temp, dtype=np.float64).ravel(), _hot_s(tm)
+
+    from MixedPrecisionIMC import Input
+    t_init = float(Input.get_scalar(inp, "T_INIT"))
+    ref = np.mean([run(64, 5000 + s, False, 2)[0] for s in range(seeds)], axis=0)
+    hot = (ref - t_init) > 0.05 * (ref - t_init).max()
+    out = {"workload": f"steps 0..{steps - 1}, FP32, {seeds} seeds each; reference FP64 plain, 2x particles, "
+                       f"{seeds} seeds; eps over the {int(hot.sum())} heated cells"}
+    for label, walk in (("plain", False), ("random_walk", True)):
+        eps2, t = [], 0.0
+        for s in range(seeds):
+            temp, hs = run(32, (6000 if walk else 7000) + s, walk)
+            eps2.append(float(np.mean(((temp[hot] - ref[hot]) / ref[hot]) ** 2)))
+            t += hs
+        e2 = float(np.mean(eps2))
+        out[label] = {"eps_rms": e2 ** 0.5, "gpu_s_per_run": t / seeds, "fom": 1.0 / (e2 * t / seeds)}
+    out["random_walk_fom_speedup"] = out["random_walk"]["fom"] / out["plain"]["fom"]
     return out
 
 
@@ -414,7 +477,8 @@ def main():
               flush=True)
     if not args.no_large_2d:
         line["large_2d"] = large_2d_leg()
-        print(f"[bench] large 2D leg: {line['large_2d']['fp32']['histories_per_s']:.4g} histories/s at FP32",
+        print(f"[bench] large 2D leg: {line['large_2d']['fp32']['histories_per_s']:.4g} histories/s at FP32, "
+              f"random walk FOM speedup {line['large_2d']['fom']['random_walk_fom_speedup']:.3g}",
               file=sys.stderr, flush=True)
     if args.precision_study:
         line["precision_study"] = precision_study()

@@ -892,7 +892,7 @@ __device__ __forceinline__ float rw_census_y(float a, float u, uint32_t slot, ui
 // P(T <= a) = 1 - sum_k 2 e^{-j_k^2 a} / (j_k J1(j_k)) (j_k the zeros of J0; an exact table of its
 // inverse CDF, with the one-term tail a = ln(c1 / (1 - u)) / j1^2), at a uniform point of the
 // circle, in a direction psi from the outward normal with density cos psi (1 + (4/pi) cos psi) / 4
-// (the P1 emergent law of the plane, as mu (1 + 1.5 mu) is in 1D).  A walker still inside at
+// (the P1 emergent law of the plane, as mu (1 + 1.5 mu) is in 1D), set as the normal turned by psi.  A walker still inside at
 // census is placed by the disk's conditional density: radius from the fundamental mode
 // rho J0(j1 rho) (a table of its inverse CDF rho J1(j1 rho) / J1(j1)) for a >= 0.2, else a
 // Gaussian of variance 2a per axis with the first-order image of the nearest boundary, accepted
@@ -934,21 +934,22 @@ __device__ __forceinline__ float2 rw2_census(float a, const float* __restrict__ 
   return p;
 }
 
-// Exit direction from the outward normal: psi in (-pi/2, pi/2) with density
-// cos psi (1 + (4/pi) cos psi) / 4, CDF (1 + sin psi + (2/pi)(psi + pi/2) + sin(2 psi) / pi) / 4,
-// inverted by Newton's method from the cosine law's inverse asin(2u - 1).
-__device__ __forceinline__ float rw2_exit_psi(float u) {
-  float psi = asinf(2.0f * u - 1.0f);
-#pragma unroll
-  for (int i = 0; i < 3; ++i) {
-    float sn, cs;
-    __sincosf(psi, &sn, &cs);
-    const float F = 0.25f * (1.0f + sn + 0.63661977f * (psi + 1.5707963f) + 0.31830989f * (2.0f * sn * cs));
-    const float pdf = 0.25f * cs * (1.0f + 1.2732395f * cs);
-    psi -= (F - u) / fmaxf(pdf, 1e-6f);
-    psi = fminf(fmaxf(psi, -1.5707963f), 1.5707963f);
+// Exit direction from the outward normal as (sin psi, cos psi), psi with density
+// cos psi (1 + (4/pi) cos psi) / 4 on (-pi/2, pi/2): proposals from the cosine law (sin psi = 2u - 1)
+// accepted with probability (1 + (4/pi) cos psi) / (1 + 4/pi) (0.88 of them); no trigonometry.
+// The first proposal uses (u, v); later ones Philox blocks 18, 19, ...  After 8 rejections the
+// last proposal is kept (probability 5e-8, a cosine-law direction).
+__device__ __forceinline__ float2 rw2_exit_dir(float u, float v, uint32_t slot, uint32_t ev, uint32_t seed,
+                                               uint32_t step) {
+  float sn = 2.0f * u - 1.0f, cs = sqrtf(fmaxf(1.0f - sn * sn, 0.0f));
+  for (uint32_t at = 0; at < 8u; ++at) {
+    if (v * 2.2732395f < 1.0f + 1.2732395f * cs) break;
+    const u4 r = philox(slot, ev, TAG_RW, 18u + at, seed, step);
+    sn = 2.0f * u24(r.r[0]) - 1.0f;
+    cs = sqrtf(fmaxf(1.0f - sn * sn, 0.0f));
+    v = u24(r.r[1]);
   }
-  return psi;
+  return make_float2(sn, cs);
 }
 
 template <int P, int TM, bool TWO_D, int ACC, bool CELL>
@@ -1038,6 +1039,9 @@ __global__ __launch_bounds__(TPB) void k_transport(SoA s, Geo g, Scalars* sc, vo
   // global atomic).  The sum is rounded once to the tally type per visit.
   int abin = -1;
   double aval = 0.0;
+  // 2D walk: an exit sets the direction (ox, oy) without forming its angle; the angle is formed
+  // (atan2) only if the particle reaches census before its next scattering or reflection.
+  bool mu_stale = false;
 
   for (;;) {
     // ---- lane refill from the wave's chunk ----
@@ -1045,7 +1049,7 @@ __global__ __launch_bounds__(TPB) void k_transport(SoA s, Geo g, Scalars* sc, vo
     while (need != 0ull && !done) {
       if (cur >= cend) {
         uint32_t base = 0;
-        constexpr uint32_t chunk = (SURR || RW) ? CHUNK_SURR : CHUNK;
+        constexpr uint32_t chunk = (SURR || (RW && !TWO_D)) ? CHUNK_SURR : CHUNK;   // measured per kernel
         if (lane == 0) base = atomicAdd(&sc->work, chunk);
         base = __shfl(base, 0, 64);
         if (base >= n) { done = true; break; }
@@ -1066,6 +1070,7 @@ __global__ __launch_bounds__(TPB) void k_transport(SoA s, Geo g, Scalars* sc, vo
         if (TWO_D) { ox = Tm<T>::cos_(mu); oy = Tm<T>::sin_(mu); }
         ev = 0;
         has = true;
+        if (RW && TWO_D) mu_stale = false;
       }
       cur += min((uint32_t)__popcll(need), avail);
       need = __ballot(!has);
@@ -1154,8 +1159,10 @@ __global__ __launch_bounds__(TPB) void k_transport(SoA s, Geo g, Scalars* sc, vo
         const T xn = x + Tm<T>::from((double)(R0 * cph)), yn = y + Tm<T>::from((double)(R0 * sph));
         x = xn < zero ? zero : (xn > cdx_ ? cdx_ : xn);
         y = yn < zero ? zero : (yn > cdy_ ? cdy_ : yn);
-        mu = Tm<T>::from((double)(phi + rw2_exit_psi(u24(q.r[2]))));
-        ox = Tm<T>::cos_(mu); oy = Tm<T>::sin_(mu);
+        const float2 sc2 = rw2_exit_dir(u24(q.r[2]), u24(q.r[3]), slot, ev, g.seed, g.step);   // (sin, cos) psi
+        ox = Tm<T>::from((double)(cph * sc2.y - sph * sc2.x));   // the normal turned by psi
+        oy = Tm<T>::from((double)(sph * sc2.y + cph * sc2.x));
+        mu_stale = true;
       } else {                                                 // census inside the disk
         const W Enew = E * (W)__expf(-sdf * s_rem);
         val = (Acc)((E - Enew) * cidxy);
@@ -1430,7 +1437,7 @@ __global__ __launch_bounds__(TPB) void k_transport(SoA s, Geo g, Scalars* sc, vo
                 } else { iy -= 1; y = dys[iy]; }
               }
             }
-            if (turned) { ox = Tm<T>::cos_(mu); oy = Tm<T>::sin_(mu); }
+            if (turned) { ox = Tm<T>::cos_(mu); oy = Tm<T>::sin_(mu); if (RW) mu_stale = false; }
           }
         }
         if (dead) {
@@ -1448,6 +1455,7 @@ __global__ __launch_bounds__(TPB) void k_transport(SoA s, Geo g, Scalars* sc, vo
             } else {
               mu = Tm<T>::from(TWO_PI_D * Tm<T>::d(Tm<T>::uscat(q, true)));
               ox = Tm<T>::cos_(mu); oy = Tm<T>::sin_(mu);
+              if (RW) mu_stale = false;
             }
           }
           if (d == dcen) {                                                    // :185-193 / :712-717
@@ -1455,7 +1463,7 @@ __global__ __launch_bounds__(TPB) void k_transport(SoA s, Geo g, Scalars* sc, vo
             CK[slot] = (uint32_t)cl | ((uint32_t)k << KSHIFT);
             X[slot] = x;
             if (TWO_D) Y[slot] = y;
-            MU[slot] = mu;
+            MU[slot] = (RW && TWO_D && mu_stale) ? Tm<T>::atan2_(oy, ox) : mu;
             TT[slot] = zero;
             EE[slot] = E;
             ++cens;

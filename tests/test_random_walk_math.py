@@ -110,7 +110,7 @@ def test_census_position_follows_the_conditional_density(a):
     assert np.max(np.abs(emp - model)) < 0.02
 
 
-# ---- the plane (imc_kernels.hip: rw2_exit_a, rw2_census, rw2_exit_psi; imc_api.cpp: DiskSeries) ----
+# ---- the plane (imc_kernels.hip: rw2_exit_a, rw2_census, rw2_exit_dir; imc_api.cpp: DiskSeries) ----
 # First exit of 2D diffusion from the centre of the unit disk, a = D t / R0^2:
 # P(T <= a) = 1 - sum_k 2 e^{-j_k^2 a} / (j_k J1(j_k)); E[a] = 1/4 and E[a^2] = 3/32 (from
 # Delta u = -1, Delta v = -2u with u = v = 0 on the rim: u = (1 - r^2)/4, v(0) = 3/32).
@@ -178,13 +178,19 @@ def disk_census(a, rad, rng, n):
     return out
 
 
-def exit_psi(u):
-    psi = np.arcsin(2.0 * u - 1.0)
-    for _ in range(3):
-        F = 0.25 * (1.0 + np.sin(psi) + (2 / np.pi) * (psi + np.pi / 2) + np.sin(2 * psi) / np.pi)
-        pdf = 0.25 * np.cos(psi) * (1.0 + (4 / np.pi) * np.cos(psi))
-        psi = np.clip(psi - (F - u) / np.maximum(pdf, 1e-6), -np.pi / 2, np.pi / 2)
-    return psi
+def exit_psi(rng, n):
+    """rw2_exit_dir: cosine-law proposals sin psi = 2u - 1, accepted with probability
+    (1 + (4/pi) cos psi) / (1 + 4/pi), at most 8 redraws."""
+    sn = 2.0 * rng.random(n) - 1.0
+    v = rng.random(n)
+    for _ in range(8):
+        cs = np.sqrt(np.maximum(1.0 - sn * sn, 0.0))
+        rej = v * (1 + 4 / np.pi) >= 1.0 + (4 / np.pi) * cs
+        if not rej.any():
+            break
+        sn[rej] = 2.0 * rng.random(int(rej.sum())) - 1.0
+        v[rej] = rng.random(int(rej.sum()))
+    return np.arcsin(sn)
 
 
 def test_disk_constants_match_the_series():
@@ -229,8 +235,7 @@ def test_disk_census_radius_follows_the_conditional_density(a):
 
 
 def test_disk_exit_direction_law():
-    u = np.random.default_rng(3).random(400_000)
-    psi = np.sort(exit_psi(u))
+    psi = np.sort(exit_psi(np.random.default_rng(3), 400_000))
     F = 0.25 * (1.0 + np.sin(psi) + (2 / np.pi) * (psi + np.pi / 2) + np.sin(2 * psi) / np.pi)
     emp = np.arange(1, psi.size + 1) / psi.size
     assert np.max(np.abs(emp - F)) < 5e-3
