@@ -206,7 +206,10 @@ int imc_surrogate_enable(imc_ctx* ctx, int32_t on);
  * one-dimensional diffusion with D = c / (3 sigma_c), sigma_c = sigma_a (1 - f) + sigma_s
  * (the reference used a sphere law and dropped sigma_s).  The weight follows the exact path
  * length and every deposit is E_in - E_out, so energy is conserved (the reference's is not,
- * quirk Q6).  Enabling it disables the surrogate and the other way round.  1D only. */
+ * quirk Q6).  In 2D (directions in the plane, D = c / (2 sigma_c)) the walk is taken over the
+ * disk of radius R0 = the distance to the nearest of the four faces, with the exit-time law of
+ * planar diffusion from the disk's centre; the disk lies inside the cell.  Enabling it
+ * disables the surrogate and the other way round. */
 int imc_random_walk_enable(imc_ctx* ctx, int32_t on, double r_min);
 
 /* ---- Planck frequency sampling -----------------------------------------------------
