@@ -227,7 +227,7 @@ DECK_CHANNEL = os.path.join(ROOT, "decks", "c3_channel_fp16.txt")
 DECK_LARGE2D = os.path.join(ROOT, "decks", "c5_blocks2d_fp16.txt")
 
 
-def large_2d_leg(warmup: int = 4, steps: int = 8):
+def large_2d_leg(warmup: int = 4, steps: int = 8, walk: bool = True):
     """The large 2D workload (1024 x 1024 cells, 64 x 64 random material blocks, NINPUT 2e7,
     NMAX 2e8; decks/c5_blocks2d_fp16.txt) on a window of its first steps.  Its deck asks for
     FP16 tracking, which the engine refuses (the cold T^-3 blocks have 1/sigma_c below binary16's
@@ -264,6 +264,8 @@ def large_2d_leg(warmup: int = 4, steps: int = 8):
                    "tally_mode": TALLY_MODES.get(tm["transport_tally_mode"]),
                    "max_abs_energy_error": max(abs(e) for e in simvars.energy_errors)}
     particles.close()
+    if not walk:
+        return out
     # the same window with the analytic random walk (at least 5 mean free paths from every face)
     mesh, simvars, particles = M.setup(inp, precision=32)
     particles.random_walk_enable(True, r_min=5.0)
@@ -392,6 +394,8 @@ def main():
     ap.add_argument("--no-surrogate", action="store_true")
     ap.add_argument("--no-host-update", action="store_true", help="skip the host-update comparison loop")
     ap.add_argument("--no-large-2d", action="store_true", help="skip the large 2D (1024 x 1024) leg")
+    ap.add_argument("--large-2d-only", action="store_true",
+                    help="run only the large 2D leg's plain FP32 window and print it as the JSON line")
     ap.add_argument("--blocks-per-cu", type=int, default=0, help="transport grid bound (0: automatic)")
     ap.add_argument("--precision-study", action="store_true",
                     help="also run the FP16/FP32/FP64 tally-error study on the 2D channel deck")
@@ -400,6 +404,9 @@ def main():
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the engine runs on a "
               f"single MI355X; multi-GPU sharding is a design note only)", file=sys.stderr)
         sys.exit(2)
+    if args.large_2d_only:
+        print(json.dumps({"large_2d": large_2d_leg(walk=False)}), flush=True)
+        return
 
     import MixedPrecisionIMC as M
     from MixedPrecisionIMC import Input

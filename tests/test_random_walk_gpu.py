@@ -147,3 +147,31 @@ def test_random_walk_2d_reproduces_plain_heating():
     assert walk_events > 0 and hot.sum() >= 4, msg
     assert ratio.max() < 1.0, msg
     assert segs[True] * 10 < segs[False], msg
+
+
+@pytest.mark.parametrize("prec", [64, 32, 16])
+@pytest.mark.parametrize("geom", ["1D", "2D"])
+def test_random_walk_every_precision_balances_energy(prec, geom):
+    """The walk kernels of every tracking precision: walk events happen and every step balances
+    energy (the walk conserves it exactly; FP16/FP32 leave the tally rounding)."""
+    import MixedPrecisionIMC as M
+    from MixedPrecisionIMC import Input
+    print(f"[random walk] {geom} FP{prec} energy balance", flush=True)
+    if geom == "1D":
+        inp = _c4_inputs(n_input=100000, n_max=1000000)
+        ts = "CONSTANT"
+    else:
+        inp = Input.readInputs(os.path.join(ROOT, "decks", "c3_channel_fp16.txt"))
+        inp["SIGMA_A_VALS"] = [5000.0] + list(inp["SIGMA_A_VALS"][1:])   # walls thick enough to walk in
+        inp["NINPUT"] = "100000"
+        inp["NMAX"] = "1000000"
+        ts = "RAMP" if str(inp["TIMESTEPPING"]).upper() == "RAMP" else "CONSTANT"
+    mesh, sv, parts = M.setup(inp, precision=prec, seed=77)
+    parts.random_walk_enable(True, r_min=3.0)
+    for _ in range(4):
+        M.step_device(inp, mesh, sv, parts, ts)
+    tm = parts.timing()
+    parts.close()
+    tol = {64: 1e-9, 32: 2e-4, 16: 2e-3}[prec]
+    assert tm["surrogate_events"] > 0
+    assert max(abs(e) for e in sv.energy_errors) < tol, sv.energy_errors
