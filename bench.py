@@ -182,16 +182,20 @@ def _run_c4(prec, n_input, seed, steps, params=None, train=False, random_walk=Fa
     return mesh.temp.copy(), hot_s, tm, params, info
 
 
-def surrogate_leg(steps=20, n_input=400000, seeds=4):
+def surrogate_leg(steps=20, n_input=400000, seeds=4, params=None, save_params=None):
     """Thick/thin blocks workload: FP32 plain vs FP32 + learned surrogate (and vs the analytic
     random walk, the non-learned filling of the same slot, r_min 5) at equal tally error
     against an FP64 GPU oracle (plain FP64, 4x the particles, 4 seeds).
-    eps = relative RMS over cells of the final material temperature; FOM = 1/(eps^2 t)."""
+    eps = relative RMS over cells of the final material temperature; FOM = 1/(eps^2 t).
+    params: trained network parameters to use instead of training one (a .npy path or an array);
+    save_params: a .npy path for the parameters the leg trained."""
     import numpy as np
+
+    if isinstance(params, str):
+        params = np.load(params)
 
     ref = np.mean([_run_c4(64, 4 * n_input, 9000 + s, steps)[0] for s in range(seeds)], axis=0)
     out = {}
-    params = None
     train_info = None
     for label in ("plain", "surrogate", "random_walk"):
         eps2, t, hist, segs, nsurr = [], 0.0, 0, 0, 0
@@ -220,6 +224,8 @@ def surrogate_leg(steps=20, n_input=400000, seeds=4):
     if train_info:
         out["training"] = {k: train_info[k] for k in ("samples", "events", "generate_s", "train_s")}
         out["training"]["final_loss"] = train_info["loss_per_epoch"][-1]
+        if save_params:
+            np.save(save_params, params)
     return out
 
 
@@ -394,6 +400,10 @@ def main():
     ap.add_argument("--no-surrogate", action="store_true")
     ap.add_argument("--no-host-update", action="store_true", help="skip the host-update comparison loop")
     ap.add_argument("--no-large-2d", action="store_true", help="skip the large 2D (1024 x 1024) leg")
+    ap.add_argument("--surrogate-only", action="store_true",
+                    help="run only the surrogate leg (thick/thin blocks) and print it as the JSON line")
+    ap.add_argument("--surrogate-params", default=None, help="use these trained parameters (.npy), no training")
+    ap.add_argument("--save-surrogate-params", default=None, help="save the parameters the surrogate leg trains (.npy)")
     ap.add_argument("--large-2d-only", action="store_true",
                     help="run only the large 2D leg's plain FP32 window and print it as the JSON line")
     ap.add_argument("--blocks-per-cu", type=int, default=0, help="transport grid bound (0: automatic)")
@@ -404,6 +414,10 @@ def main():
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the engine runs on a "
               f"single MI355X; multi-GPU sharding is a design note only)", file=sys.stderr)
         sys.exit(2)
+    if args.surrogate_only:
+        print(json.dumps({"surrogate": surrogate_leg(params=args.surrogate_params,
+                                                     save_params=args.save_surrogate_params)}), flush=True)
+        return
     if args.large_2d_only:
         print(json.dumps({"large_2d": large_2d_leg(walk=False)}), flush=True)
         return
