@@ -35,6 +35,20 @@ HBM_PEAK_GBS = 8000.0
 TALLY_MODES = {0: "WAVEROW", 1: "BLOCKROW", 2: "GLOBAL", 3: "HASH"}
 
 
+def profiled_valu_busy():
+    """Fraction of the SIMDs' issue cycles the transport kernel keeps busy with VALU work on this
+    workload, from the committed SQ counter summary (the kernel is issue-bound, not HBM-bound)."""
+    path = os.path.join(ROOT, "profiles", "transport_marshak_valu.md")
+    try:
+        with open(path) as f:
+            for ln in f:
+                if ln.startswith("valu_busy_fraction:"):
+                    return float(ln.split(":", 1)[1]), os.path.relpath(path, ROOT)
+    except OSError:
+        pass
+    return None, None
+
+
 def profiled_traffic():
     """HBM bytes per transport launch on this workload, from the committed rocprofv3 counter
     summary (FETCH_SIZE and WRITE_SIZE passes, scaled by a plain copy's readings)."""
@@ -453,6 +467,7 @@ def main():
     alg_bytes = sbytes * (hist + tm["census_out"])
     achieved = alg_bytes / kern_s / 1e9 if kern_s > 0 else 0.0
     traffic, traffic_src = profiled_traffic()
+    valu_busy, valu_src = profiled_valu_busy()
     line = {
         "metric": "histories/sec at equal tally error vs FP64 oracle; surrogate vs plain speedup",
         "value": hist / wall,
@@ -473,6 +488,7 @@ def main():
                    "state_bytes": sbytes},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "traffic_source": traffic_src,
+                     "valu_busy": valu_busy, "valu_busy_source": valu_src,
                      "kernel": f"k_transport<32,{TALLY_MODES.get(tm['transport_tally_mode'])},1D>",
                      "grid_blocks": tm["transport_grid"],
                      "avg_launch_ms": tm["ms_transport_kernel"] / launches,
