@@ -286,7 +286,6 @@ __global__ void k_counts(int nx, int ny, int C, int64_t NE, int quirks, int64_t 
   ent_e[e] = n_div > 0 ? (double)((W)en / (W)(double)n_div) : 0.0;
 }
 
-// Single-block exclusive scan of the birth counts; sets n_births / n_total.
 // ---------------------------------------------------------------------------------
 // Exclusive scan of int64 counts over many blocks: tile sums (k_scan_tile_sums), a
 // single-block scan of the tile sums (k_exscan_i64, with the grand total left in entry
