@@ -173,7 +173,7 @@ def _c4(prec, n_input):
     return inp
 
 
-def _run_c4(prec, n_input, seed, steps, params=None, train=False, random_walk=False):
+def _run_c4(prec, n_input, seed, steps, params=None, train=False, random_walk=False, rw_min=8.0):
     import MixedPrecisionIMC as M
     from MixedPrecisionIMC import Surrogate
 
@@ -181,7 +181,7 @@ def _run_c4(prec, n_input, seed, steps, params=None, train=False, random_walk=Fa
     mesh, simvars, particles = M.setup(inp, seed=seed)
     info = None
     if random_walk:
-        particles.random_walk_enable(True)
+        particles.random_walk_enable(True, r_min=rw_min)
     elif train:
         info = Surrogate.train(particles, seed=seed)
         params = particles.surrogate_params()
@@ -198,7 +198,7 @@ def _run_c4(prec, n_input, seed, steps, params=None, train=False, random_walk=Fa
 
 def surrogate_leg(steps=20, n_input=400000, seeds=8, ref_seeds=4, params=None, save_params=None):
     """Thick/thin blocks workload: FP32 plain vs FP32 + learned surrogate (and vs the analytic
-    random walk, the non-learned filling of the same slot, r_min 5) at equal tally error
+    random walk, the non-learned filling of the same slot, r_min 8) at equal tally error
     against an FP64 GPU oracle (plain FP64, 4x the particles, ref_seeds seeds); `seeds` runs per
     method (each run's error is one draw of a noisy estimate: 8 seeds halve the variance of the
     measured FOM ratio against 4).
@@ -288,9 +288,9 @@ def large_2d_leg(warmup: int = 4, steps: int = 8, walk: bool = True):
     particles.close()
     if not walk:
         return out
-    # the same window with the analytic random walk (at least 5 mean free paths from every face)
+    # the same window with the analytic random walk (at least 8 mean free paths from every face)
     mesh, simvars, particles = M.setup(inp, precision=32)
-    particles.random_walk_enable(True, r_min=5.0)
+    particles.random_walk_enable(True, r_min=8.0)
     for _ in range(warmup):
         M.step_device(inp, mesh, simvars, particles, ts)
     particles.synchronize()
@@ -312,7 +312,7 @@ def large_2d_leg(warmup: int = 4, steps: int = 8, walk: bool = True):
     return out
 
 
-def large_2d_fom(inp, steps: int = 8, seeds: int = 3):
+def large_2d_fom(inp, steps: int = 8, seeds: int = 3, r_min: float = 8.0):
     """Random walk against plain transport on the large 2D workload at equal error: after `steps`
     steps, eps = relative RMS of the material temperature over the heated cells (rise over T_INIT
     above 5% of the largest) against an FP64 plain reference (2x the particles, `seeds` seeds);
@@ -329,7 +329,7 @@ def large_2d_fom(inp, steps: int = 8, seeds: int = 3):
         d["NMAX"] = str(10 * scale * n_input)
         mesh, simvars, particles = M.setup(d, precision=prec, seed=seed, capacity=3 * scale * n_input)
         if walk:
-            particles.random_walk_enable(True, r_min=5.0)
+            particles.random_walk_enable(True, r_min=r_min)
         for _ in range(steps):
             M.step_device(d, mesh, simvars, particles, ts)
         tm = particles.timing()

@@ -412,7 +412,7 @@ class Engine:
         self._check(lib().imc_sample_planck(self._h, t.size, _dp(t), int(stream) & 0xFFFFFFFF, _dp(out)))
         return out
 
-    def random_walk_enable(self, on: bool = True, r_min: float = 5.0):
+    def random_walk_enable(self, on: bool = True, r_min: float = 8.0):
         """imc_random_walk_enable: the analytic random walk in cells at least r_min collision
         mean free paths deep (the Transport.MC_RW slot); disables the surrogate."""
         self._check(lib().imc_random_walk_enable(self._h, int(bool(on)), float(r_min)))

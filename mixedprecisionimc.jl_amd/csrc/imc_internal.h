@@ -159,7 +159,7 @@ struct Ctx {
 
   // analytic random walk (imc_random_walk_enable); table of RW_TABLE_POINTS values
   bool rw_enabled = false;
-  double rw_min = 5.0;
+  double rw_min = 8.0;
   float* d_rw_table = nullptr;       // [4096] inverse CDF of the dimensionless exit time
 
   // host copies of the last transport tallies and census tally

@@ -262,6 +262,18 @@ function step!(e::Engine, mesh, simvars)
 end
 
 """
+    random_walk_enable!(e::Engine, on::Bool = true; r_min = 8.0)
+
+The analytic random walk in the `Transport.MC_RW` slot (imc_transport.jl:212-479, corrected):
+in 1D over intervals, in 2D over disks, at least `r_min` collision mean free paths from every face
+of the cell (`imc_random_walk_enable`).
+"""
+function random_walk_enable!(e::Engine, on::Bool = true; r_min::Real = 8.0)
+    check(e, ccall((:imc_random_walk_enable, LIB), Cint, (Ptr{Cvoid}, Int32, Float64), e.ptr, Int32(on), Float64(r_min)))
+    return nothing
+end
+
+"""
     sample_planck(e::Engine, temps::AbstractVector; stream = 0)
 
 Planck frequencies `temps[i] * x_i` (`Sourcing.sample_planck`, imc_sourcing.jl:372-399, with
