@@ -196,12 +196,12 @@ def _run_c4(prec, n_input, seed, steps, params=None, train=False, random_walk=Fa
     return mesh.temp.copy(), hot_s, tm, params, info
 
 
-def surrogate_leg(steps=20, n_input=400000, seeds=8, ref_seeds=4, params=None, save_params=None):
+def surrogate_leg(steps=20, n_input=400000, seeds=12, ref_seeds=4, params=None, save_params=None):
     """Thick/thin blocks workload: FP32 plain vs FP32 + learned surrogate (and vs the analytic
     random walk, the non-learned filling of the same slot, r_min 8) at equal tally error
     against an FP64 GPU oracle (plain FP64, 4x the particles, ref_seeds seeds); `seeds` runs per
-    method (each run's error is one draw of a noisy estimate: 8 seeds halve the variance of the
-    measured FOM ratio against 4).
+    method (each run's error is one draw of a noisy estimate: 12 seeds cut the variance of the
+    measured FOM ratio to a third of 4 seeds').
     eps = relative RMS over cells of the final material temperature; FOM = 1/(eps^2 t).
     params: trained network parameters to use instead of training one (a .npy path or an array);
     save_params: a .npy path for the parameters the leg trained."""

@@ -184,7 +184,7 @@ typedef struct {
   float tau_b_min, tau_b_max;         /* path budget in mean free paths */
   float kappa_min, kappa_max;         /* f*sigma_a / sigma_c */
   float depth_min;                    /* applied only this many mean free paths from both faces */
-} imc_surrogate_box;
+} imc_surrogate_box;   /* default (before any configure): {2.5, 200, 2, 1e4, 1e-5, 4, 0.7} */
 
 int imc_surrogate_configure(imc_ctx* ctx, const imc_surrogate_box* box);
 int imc_surrogate_init(imc_ctx* ctx, uint32_t seed);

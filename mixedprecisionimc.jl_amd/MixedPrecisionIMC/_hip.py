@@ -351,7 +351,7 @@ class Engine:
         return inc
 
     # -- learned thick-cell surrogate (slot of Transport.MC_RW) -----------------
-    def surrogate_configure(self, tau_cell=(20.0, 200.0), tau_b=(2.0, 1e4), kappa=(1e-5, 4.0), depth_min=3.0):
+    def surrogate_configure(self, tau_cell=(2.5, 200.0), tau_b=(2.0, 1e4), kappa=(1e-5, 4.0), depth_min=0.7):
         box = SurrogateBox(tau_cell[0], tau_cell[1], tau_b[0], tau_b[1], kappa[0], kappa[1], depth_min)
         self._check(lib().imc_surrogate_configure(self._h, ctypes.byref(box)))
 

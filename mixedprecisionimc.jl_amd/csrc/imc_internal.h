@@ -141,7 +141,7 @@ struct Ctx {
   int64_t tile_len = 0;
 
   // learned thick-cell surrogate
-  SurrBox surr_box{20.f, 200.f, 2.f, 1e4f, 1e-5f, 4.f, 3.f};
+  SurrBox surr_box{2.5f, 200.f, 2.f, 1e4f, 1e-5f, 4.f, 0.7f};   // the tuned default box
   bool surr_enabled = false;
   int64_t surr_cap = 0, surr_n = 0;
   float* d_surr_feat = nullptr;      // [surr_cap][6]
