@@ -11,7 +11,7 @@ import numpy as np
 
 
 def tointeger(x: float) -> int:
-    """imc_utilities.jl:6-21.  Counts are 64-bit in this build (the Float16 -> UInt16 overflow
+    """imc_utilities.jl:6-21.  Counts are 64-bit here (the Float16 -> UInt16 overflow
     of the reference for values above 65504 does not occur)."""
     xi = int(round(float(x)))
     if float(xi) != float(x):

@@ -20,11 +20,12 @@ pytestmark = pytest.mark.gpu
 
 
 def _dump(name, obj):
-    """Keeps the raw numbers of a statistical test for inspection (gpurun_out/ when present)."""
+    """Keeps the raw numbers of a statistical test for inspection, in the folder named by the
+    IMC_TEST_OUTPUT_DIR environment variable when it is set."""
     import json
     import os
-    d = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "gpurun_out")
-    if os.path.isdir(d):
+    d = os.environ.get("IMC_TEST_OUTPUT_DIR", "")
+    if d and os.path.isdir(d):
         with open(os.path.join(d, name + ".json"), "w") as f:
             json.dump(obj, f)
 
