@@ -32,6 +32,8 @@ CASES = {
     "slab_linearized_3scales": lambda: _deck("c1_slab_fp64.txt", NINPUT=20000, NMAX=200000,
                                              ENERGYSCALES=[1024.0, 8.0, 1.0]),
     "channel2d": lambda: _deck("c3_channel_fp16.txt", NINPUT=200000, NMAX=1000000),
+    "channel2d_2scales": lambda: _deck("c3_channel_fp16.txt", NINPUT=100000, NMAX=500000,
+                                       ENERGYSCALES=[64.0, 1.0]),
 }
 RTOL = 1e-12
 STEPS = 4
