@@ -1791,6 +1791,9 @@ hipError_t launch_clean(Ctx& c, int64_t n_total) {
   if (err != hipSuccess) return err;
   int grid, tm; size_t lds;
   clean_choose(c, &grid, &tm, &lds);
+  // no more blocks than tiles (fewer partial rows to reduce); the partials are sized for the
+  // full grid
+  grid = (int)std::max<int64_t>(1, std::min<int64_t>(grid, ntiles));
   if (tm == TM_GLOBAL) {
     err = launch_zero(c, c.d_rad, sizeof(double) * c.C);
     if (err != hipSuccess) return err;
