@@ -320,8 +320,9 @@ __global__ __launch_bounds__(SCAN_TPB) void k_scan_tile_sums(const int64_t* __re
   if (threadIdx.x == 0) tsum[blockIdx.x] = s;
 }
 
-__global__ __launch_bounds__(SCAN_TPB) void k_scan_tiles(const int64_t* a, int64_t n, const int64_t* __restrict__ toff,
-                                                          int64_t* out) {
+// single: one tile, scanned from 0, whose total goes to toff[1] (the tile-sum pass is skipped).
+__global__ __launch_bounds__(SCAN_TPB) void k_scan_tiles(const int64_t* a, int64_t n, int64_t* toff, int64_t* out,
+                                                          int single) {
   __shared__ int64_t tile[SCAN_TILE];
   __shared__ int64_t wtot[SCAN_TPB / 64];
   const int64_t base = (int64_t)blockIdx.x * SCAN_TILE;
@@ -344,7 +345,7 @@ __global__ __launch_bounds__(SCAN_TPB) void k_scan_tiles(const int64_t* a, int64
   }
   if (lane == 63) wtot[w] = incl;
   __syncthreads();
-  int64_t run = toff[blockIdx.x] + incl - sum;
+  int64_t run = (single ? 0 : toff[blockIdx.x]) + incl - sum;
   for (int j = 0; j < w; ++j) run += wtot[j];
 #pragma unroll
   for (int j = 0; j < SCAN_PER; ++j) {
@@ -352,6 +353,7 @@ __global__ __launch_bounds__(SCAN_TPB) void k_scan_tiles(const int64_t* a, int64
     mine[j] = run;
     run += v;
   }
+  if (single && threadIdx.x == SCAN_TPB - 1) toff[1] = run;   // the last thread's end is the total
   __syncthreads();
 #pragma unroll
   for (int j = 0; j < SCAN_PER; ++j) {
@@ -366,9 +368,13 @@ __global__ __launch_bounds__(1024) void k_exscan_i64(int64_t* __restrict__ a, in
 // scan_tiles(n) + 1 entries.  n >= 1.
 hipError_t launch_scan_i64(Ctx& c, const int64_t* a, int64_t n, int64_t* out, int64_t* tiles) {
   const int64_t nt = scan_tiles(n);
+  if (nt == 1) {                                             // one launch for a single tile
+    k_scan_tiles<<<1, SCAN_TPB, 0, c.stream>>>(a, n, tiles, out, 1);
+    return hipGetLastError();
+  }
   k_scan_tile_sums<<<(unsigned)nt, SCAN_TPB, 0, c.stream>>>(a, n, tiles);
   k_exscan_i64<<<1, 1024, 0, c.stream>>>(tiles, nt + 1);     // entry nt (any value) ends as the total
-  k_scan_tiles<<<(unsigned)nt, SCAN_TPB, 0, c.stream>>>(a, n, tiles, out);
+  k_scan_tiles<<<(unsigned)nt, SCAN_TPB, 0, c.stream>>>(a, n, tiles, out, 0);
   return hipGetLastError();
 }
 
